@@ -1,0 +1,162 @@
+"""Synthetic, seeded initial conditions for the hot-path workloads.
+
+The reference builds its initial conditions with one compile-time `setup`
+routine per problem (src/setup_unifdisND.f90, src/setup_shockND.f90,
+src/setup_wave_x_ND_dust.f90); this module generates the same kinds of
+particle sets directly as arrays, so that the CPU oracle, the compiled
+reference harness and the GPU library all start from bit-identical inputs.
+
+Smoothing lengths start at h = hfact (m/rho)^(1/ndim), as the reference's
+primitive2conservative does (src/conservative2primitive.f90:519-523).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .options import Options, empty_state
+
+
+def _h_from_rho(pmass, rho, hfact, ndim):
+    return hfact * (pmass / rho) ** (1.0 / ndim)
+
+
+def gaussian_velocity_field(n: int, ndim: int, seed: int, slope: float = -4.0,
+                            vrms: float = 1.0) -> np.ndarray:
+    """Random velocity on an n^ndim lattice with |v(k)|^2 ~ k^slope for
+    1 <= |k| <= n/2, zero mean, rescaled to the requested rms speed.
+    Returns shape (n**ndim, 3) in lattice order (x fastest)."""
+    rng = np.random.default_rng(seed)
+    shape = (n,) * ndim
+    k1 = np.fft.fftfreq(n) * n
+    grids = np.meshgrid(*([k1] * ndim), indexing="ij")
+    kmag = np.sqrt(sum(g * g for g in grids))
+    amp = np.zeros(shape)
+    band = (kmag >= 1.0) & (kmag <= n / 2)
+    amp[band] = kmag[band] ** (slope / 2.0)
+    out = np.empty((n ** ndim, 3))
+    for c in range(3):
+        white = rng.standard_normal(shape)
+        f = np.fft.fftn(white) * amp
+        v = np.real(np.fft.ifftn(f))
+        # ij-indexing: axis 0 is x; lattice order below has x fastest
+        out[:, c] = np.transpose(v).reshape(-1)
+    out -= out.mean(axis=0)
+    rms = math.sqrt(np.mean(np.sum(out * out, axis=1)))
+    out *= vrms / rms
+    return out
+
+
+def periodic_mhd_box(nside: int, ndim: int = 3, seed: int = 1,
+                     jitter: float = 0.1, vrms: float = 1.0,
+                     B0=(0.1, 0.0, 0.0), gamma: float = 5.0 / 3.0,
+                     hfact: float = 1.2, u0: float = 0.9,
+                     vel_seed: int = 2, opts: Options | None = None):
+    """Periodic unit box of nside^ndim particles on a cubic lattice perturbed
+    by uniform random offsets of up to `jitter` lattice spacings, rho = 1,
+    Gaussian random velocity field with P(k) ~ k^-4 at the given rms speed,
+    uniform B, thermal energy u0 (P = (gamma-1) rho u0; u0 = 0.9 at
+    gamma = 5/3 gives c_s = 1).  Template: src/setup_unifdisND.f90."""
+    n = nside ** ndim
+    dx = 1.0 / nside
+    rng = np.random.default_rng(seed)
+    idx = np.indices((nside,) * ndim).reshape(ndim, -1)[::-1].T  # x fastest
+    x = (idx + 0.5) * dx + rng.uniform(-jitter * dx, jitter * dx, size=(n, ndim))
+    st = empty_state(n, ndim)
+    st["x"][:] = x
+    st["pmass"][:] = 1.0 / n
+    st["hh"][:] = _h_from_rho(1.0 / n, 1.0, hfact, ndim)
+    st["en"][:] = u0
+    st["vel"][:] = gaussian_velocity_field(nside, ndim, vel_seed, vrms=vrms) if vrms > 0 else 0.0
+    st["Bevol"][:] = np.asarray(B0, dtype=np.float64)
+    o = opts or Options()
+    o = o.replace(ndim=ndim, ikernel=0, iener=2, imhd=11, imagforce=2,
+                  idivbzero=2, iav=2, iavlim=(2, 1, 0), idust=0,
+                  ibound=(3, 3, 3), hfact=hfact, gamma=gamma,
+                  xmin=(0.0, 0.0, 0.0), xmax=(1.0, 1.0, 1.0), psep=dx)
+    st["alpha"][:, 0] = o.alphamin
+    st["alpha"][:, 1] = o.alphaumin
+    st["alpha"][:, 2] = o.alphaBmin
+    return o, st
+
+
+def dusty_box(nside: int, ndim: int = 2, seed: int = 3, amp: float = 1.0e-2,
+              eps_median: float = 0.1, eps_sigma: float = 0.5,
+              tstop: float = 1.0e-3, hfact: float = 1.2,
+              opts: Options | None = None):
+    """Periodic one-fluid dusty gas: nside^ndim particles, gas density
+    1 + amp sin(2 pi x) realised by moving lattice columns in the mass
+    coordinate (as src/setup_wave_x_ND_dust.f90:200-258 does), dust fraction
+    eps lognormal (median eps_median, sigma eps_sigma, clipped to
+    [0.01, 0.9]), particle mass m = m_gas/(1-eps)
+    (src/setup_wave_x_ND_dust.f90:150), isothermal c_s = 1, quintic kernel,
+    constant stopping time."""
+    n = nside ** ndim
+    dx = 1.0 / nside
+    rng = np.random.default_rng(seed)
+    idx = np.indices((nside,) * ndim).reshape(ndim, -1)[::-1].T
+    x = (idx + 0.5) * dx
+    # mass coordinate of rho(x) = 1 + amp sin(2 pi x):
+    # M(x) = x - amp/(2 pi) (cos(2 pi x) - 1); invert by Newton
+    xq = x[:, 0].copy()
+    target = xq.copy()
+    for _ in range(50):
+        f = xq - amp / (2 * math.pi) * (np.cos(2 * math.pi * xq) - 1.0) - target
+        fp = 1.0 + amp * np.sin(2 * math.pi * xq)
+        xq -= f / fp
+    x[:, 0] = xq
+    eps = np.clip(eps_median * np.exp(eps_sigma * rng.standard_normal(n)), 0.01, 0.9)
+    st = empty_state(n, ndim)
+    st["x"][:] = x
+    mgas = 1.0 / n
+    st["pmass"][:] = mgas / (1.0 - eps)
+    rho_tot = (1.0 + amp * np.sin(2 * math.pi * xq)) / (1.0 - eps)
+    st["hh"][:] = _h_from_rho(st["pmass"], rho_tot, hfact, ndim)
+    st["dustevol"][:] = eps
+    o = opts or Options()
+    o = o.replace(ndim=ndim, ikernel=3, iener=0, gamma=1.0, polyk=1.0,
+                  imhd=0, idivbzero=0, iav=2, iavlim=(2, 1, 0), idust=4,
+                  idrag_nature=2, Kdrag=tstop, ibound=(3, 3, 3), hfact=hfact,
+                  xmin=(0.0, 0.0, 0.0), xmax=(1.0, 1.0, 1.0), psep=dx)
+    st["alpha"][:, 0] = o.alphamin
+    st["alpha"][:, 1] = o.alphaumin
+    st["alpha"][:, 2] = o.alphaBmin
+    return o, st
+
+
+def brio_wu_1d(npart_target: int = 800, hfact: float = 1.2,
+               states=None, gamma: float = 2.0, opts: Options | None = None):
+    """1D MHD shock tube (Brio & Wu) with equal-mass particles on two
+    lattices: x in [-0.5, 0.5], left rho=1 p=1 By=1, right rho=0.125 p=0.1
+    By=-1, Bx=0.75, v=0, gamma=2, reflecting walls
+    (src/setup_shockND.f90:200-330; states as multi/multi_shock.f90:66-82).
+    `states` = (rhoL, rhoR, pL, pR, ByL, ByR, Bx) overrides the defaults."""
+    rhoL, rhoR, pL, pR, ByL, ByR, Bx = states or (1.0, 0.125, 1.0, 0.1, 1.0, -1.0, 0.75)
+    # equal masses: nL/nR = rhoL/rhoR over equal lengths
+    nL = int(round(npart_target * rhoL / (rhoL + rhoR)))
+    nR = max(2, npart_target - nL)
+    psepL = 0.5 / nL
+    psepR = 0.5 / nR
+    xL = -0.5 + (np.arange(nL) + 0.5) * psepL
+    xR = (np.arange(nR) + 0.5) * psepR
+    n = nL + nR
+    st = empty_state(n, 1)
+    st["x"][:, 0] = np.concatenate([xL, xR])
+    m = rhoL * 0.5 / nL
+    st["pmass"][:] = m
+    rho = np.concatenate([np.full(nL, rhoL), np.full(nR, rhoR)])
+    pr = np.concatenate([np.full(nL, pL), np.full(nR, pR)])
+    st["hh"][:] = _h_from_rho(m, rho, hfact, 1)
+    st["en"][:] = pr / ((gamma - 1.0) * rho)
+    st["Bevol"][:, 0] = Bx
+    st["Bevol"][:, 1] = np.concatenate([np.full(nL, ByL), np.full(nR, ByR)])
+    o = opts or Options()
+    o = o.replace(ndim=1, ikernel=0, iener=2, imhd=11, imagforce=2,
+                  idivbzero=2, iav=2, iavlim=(2, 1, 0), idust=0,
+                  ibound=(2, 0, 0), hfact=hfact, gamma=gamma,
+                  xmin=(-0.5, 0.0, 0.0), xmax=(0.5, 0.0, 0.0), psep=psepL)
+    st["alpha"][:, 0] = o.alphamin
+    st["alpha"][:, 1] = o.alphaumin
+    st["alpha"][:, 2] = o.alphaBmin
+    return o, st

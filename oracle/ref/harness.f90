@@ -1,0 +1,269 @@
+!------------------------------------------------------------------------------
+! Test-only driver for the compiled reference (ndspmhd src/, built unmodified
+! from the reference tree by oracle/ref/Makefile).  It is the oracle's
+! checker: it writes golden vectors for tests/golden, nothing else uses it.
+!
+! It plays the part of the reference's main program: it fills the module
+! globals the way initialise (src/initialiseND_mhd.f90:140-300) does for the
+! options read from an input file, calls the reference's own derivs
+! (src/derivs.f90:27) once, and writes the results.  Mode 2 additionally
+! enumerates neighbour sets with the reference's link list
+! (src/linkND.f90:45, src/get_neighbour_lists.f90:176) and the distance tests
+! of density_partial (src/density_sums.f90:499-528) and get_rates
+! (src/ratesND_mhd.f90:415).
+!
+! The reference's quit (src/ndspmhd.f90:369) lives beside its main program,
+! which this driver replaces, so the driver supplies its own quit that stops
+! with a non-zero status.
+!
+! Input / output: little-endian stream files, layout documented in
+! tests/golden/refio.py.
+!------------------------------------------------------------------------------
+program ndsp_ref_harness
+ use dimen_mhd
+ use debug
+ use loguns
+ use artvi
+ use bound
+ use eos
+ use hterms
+ use linklist
+ use options
+ use part
+ use part_in
+ use rates
+ use derivB
+ use timestep
+ use setup_params
+ use kernels,        only:setkernels,radkern,radkern2
+ use mem_allocation, only:alloc
+ use dust,           only:init_drag
+ use get_neighbour_lists, only:get_neighbour_list_partial
+ implicit none
+ integer, parameter :: maxopt = 32
+ integer :: iopt(maxopt)
+ real    :: dopt(maxopt)
+ integer :: magic,mode,np,ndimin,ierr,ierr1,ierr2,i,j,n,k,nneigh,icode,ip
+ integer :: npg,nps
+ integer, allocatable :: listneigh(:),pg(:,:),ps(:,:)
+ real, allocatable :: dustin(:),zeros(:),zeros3(:,:)
+ real :: rij2,q2i,q2j,hi1,hi21,hj1
+ real, dimension(ndim) :: dx
+ character(len=256) :: infile,outfile
+
+ call get_command_argument(1,infile)
+ call get_command_argument(2,outfile)
+ iprint = 6
+ trace = .false.
+ idebug = '      '
+ call logun
+
+ open(unit=21,file=trim(infile),access='stream',form='unformatted',status='old',iostat=ierr)
+ if (ierr /= 0) error stop 'cannot open input'
+ read(21) magic,mode,np,ndimin
+ if (magic /= 1313100880) error stop 'bad magic'
+ if (ndimin /= ndim) error stop 'input ndim does not match this build'
+ read(21) iopt
+ read(21) dopt
+!
+!--options: defaults first (src/defaults.f90:29), then the input values
+!
+ call set_default_options
+ ikernel      = iopt(1)
+ ikernelalt   = ikernel
+ iener        = iopt(2)
+ imhd         = iopt(3)
+ imagforce    = iopt(4)
+ idivbzero    = iopt(5)
+ iav          = iopt(6)
+ iavlim(1:3)  = iopt(7:9)
+ idust        = iopt(10)
+ idrag_nature = iopt(11)
+ ibound(1:ndim) = iopt(12:11+ndim)
+ maxdensits   = iopt(15)
+ idustevol    = iopt(16)
+ ivisc = 0; iexternal_force = 0; igravity = 0; iresist = 0; iambipolar = 0
+ icty = 0; ikernav = 3; ihvar = 2; iprterm = 0; ixsph = 0; iquantum = 0
+ ind_timesteps = 0; nsubsteps_divB = 0; iuse_exact_derivs = 0; idiffuse = 0
+ ibiascorrection = 0; icompute_d2v = 0; usenumdens = .false.; damp = 0.
+!--nsteps_remap is only read for imhd = 10, 20, -3, but flang evaluates both
+!  operands of the .and. guarding mod(nsteps,nsteps_remap)
+!  (src/conservative2primitive.f90:138-139), so 0 would divide by zero
+ nsteps_remap = 1
+ hfact        = dopt(1)
+ tolh         = dopt(2)
+ gamma        = dopt(3)
+ polyk        = dopt(4)
+ Kdrag        = dopt(5)
+ alphamin     = dopt(6)
+ alphaumin    = dopt(7)
+ alphabmin    = dopt(8)
+ beta         = dopt(9)
+ avdecayconst = dopt(10)
+ psidecayfact = dopt(11)
+ C_cour       = dopt(12)
+ C_force      = dopt(13)
+ xmin(1:ndim) = dopt(14:13+ndim)
+ xmax(1:ndim) = dopt(17:16+ndim)
+ psep         = dopt(20)
+ dt           = dopt(24)
+ nsteps = 0
+ time = 0.
+ onef_dust = (idust==1 .or. idust==3 .or. idust==4)
+ dndim = 1./ndim
+ rhomin = 0.
+ h_min = 0.
+ vsig2max = 0.
+ pext = 0.
+ geom = 'cartesian'
+!--as in src/initialiseND_mhd.f90:166-172
+ if (abs(gamma-1.).gt.1.e-3) then
+    avfact = log(4.)/(log((gamma+1.)/(gamma-1.)))
+ else
+    avfact = 1.0
+ endif
+ call setkernels(ikernel,ikernelalt,ndim,ierr1,ierr2)
+ if (ierr1 /= 0 .or. ierr2 /= 0) error stop 'kernel setup failed'
+ if (idust /= 0) then
+    call init_drag(ierr,gamma)
+    if (ierr /= 0) error stop 'init_drag failed'
+ endif
+!
+!--particles
+!
+ call alloc(np)
+ npart = np
+ ntotal = np
+ Bconst(:) = dopt(21:23)
+ read(21) x(1:ndim,1:np)
+ read(21) vel(1:3,1:np)
+ read(21) pmass(1:np)
+ read(21) hh(1:np)
+ read(21) en(1:np)
+ read(21) Bevol(1:3,1:np)
+ read(21) alpha(1:3,1:np)
+ read(21) psi(1:np)
+ allocate(dustin(np),zeros(np),zeros3(3,np))
+ zeros = 0.
+ zeros3 = 0.
+ read(21) dustin(1:np)
+ if (onef_dust) dustevol(1,1:np) = dustin(1:np)
+ read(21) itype(1:np)
+ close(21)
+ rho(1:np) = 0.
+ uu(1:np) = en(1:np)
+ Bfield(:,1:np) = Bevol(:,1:np)
+ if (imhd == 0) then
+    Bfield = 0.
+    Bevol = 0.
+ endif
+ if (onef_dust) then
+    dustfrac(:,1:np) = dustevol(:,1:np)
+    deltav = 0.
+ endif
+ sqrtg = 1.
+ divB = 0.
+ curlB = 0.
+ gradpsi = 0.
+ force = 0.
+ dBevoldt = 0.
+ drhodt = 0.
+ dhdt = 0.
+!
+!--the hot path: one derivative evaluation
+!
+ call derivs
+!
+!--results
+!
+ open(unit=22,file=trim(outfile),access='stream',form='unformatted',status='replace')
+ write(22) npart,ntotal,itsdensity,0
+ write(22) dtcourant,dtforce,dtdrag,dtav,vsig2max,hhmax
+ write(22) x(1:ndim,1:np)
+ write(22) hh(1:np)
+ write(22) rho(1:np)
+ write(22) gradh(1:np)
+ write(22) gradgradh(1:np)
+ write(22) drhodt(1:np)
+ write(22) dhdt(1:np)
+ write(22) dens(1:np)
+ write(22) pr(1:np)
+ write(22) spsound(1:np)
+ write(22) uu(1:np)
+ write(22) psi(1:np)
+ write(22) force(1:3,1:np)
+ write(22) dudt(1:np)
+ write(22) dendt(1:np)
+ write(22) dBevoldt(1:3,1:np)
+ write(22) daldt(1:3,1:np)
+ write(22) dpsidt(1:np)
+ write(22) divB(1:np)
+ write(22) curlB(1:3,1:np)
+ write(22) gradpsi(1:3,1:np)
+ if (onef_dust) then
+    write(22) dustfrac(1,1:np)
+    write(22) rhogas(1:np)
+    write(22) rhodust(1,1:np)
+    write(22) ddustevoldt(1,1:np)
+    write(22) deltav(1:3,1:np)
+ else
+    write(22) zeros,zeros,zeros,zeros,zeros3
+ endif
+ write(22) numneigh(1:np)
+
+ if (mode == 2) then
+!
+!--neighbour sets of the real particles, found through the reference's own
+!  cells (full stencil) with the gather test of density_partial and the
+!  symmetric test of get_rates, at the h the iteration finished with
+!
+    allocate(listneigh(ntotal),pg(3,400*np+64),ps(3,400*np+64))
+    npg = 0
+    nps = 0
+    do i=1,npart
+       call get_neighbour_list_partial(iamincell(i),listneigh,nneigh)
+       hi1 = 1./hh(i)
+       hi21 = hi1*hi1
+       do n=1,nneigh
+          j = listneigh(n)
+          dx(:) = x(:,i) - x(:,j)
+          rij2 = dot_product(dx,dx)
+          q2i = rij2*hi21
+          hj1 = 1./hh(j)
+          q2j = rij2*hj1*hj1
+          ip = j
+          icode = 0
+          if (j > npart) then
+             ip = ireal(j)
+             do k=ndim,1,-1
+                icode = 3*icode
+                if (x(k,j) > x(k,ip)) icode = icode + 1
+                if (x(k,j) < x(k,ip)) icode = icode + 2
+             enddo
+          endif
+          if (q2i < radkern2) then
+             npg = npg + 1
+             if (npg > size(pg,2)) error stop 'pair buffer overflow'
+             pg(:,npg) = (/i,ip,icode/)
+          endif
+          if (q2i < radkern2 .or. q2j < radkern2) then
+             nps = nps + 1
+             if (nps > size(ps,2)) error stop 'pair buffer overflow'
+             ps(:,nps) = (/i,ip,icode/)
+          endif
+       enddo
+    enddo
+    write(22) npg
+    write(22) pg(1:3,1:npg)
+    write(22) nps
+    write(22) ps(1:3,1:nps)
+ endif
+ close(22)
+
+end program ndsp_ref_harness
+
+subroutine quit
+ use loguns, only:iprint
+ write(iprint,*) 'ndsp_ref_harness: reference called quit'
+ stop 3
+end subroutine quit

@@ -1,0 +1,89 @@
+"""Writes the golden vectors in tests/golden/*.npz from the compiled
+reference (oracle/_ref/ndsp_ref3d, built by `make -C oracle/ref` from the
+unmodified reference sources).  Run in the container that has the
+reference tree; the GPU box only reads the committed .npz files.
+
+    python tests/golden/make_golden.py
+
+Each fixture holds the inputs (options + particle state) and the reference's
+outputs after one derivs call (src/derivs.f90:27): every field listed in
+refio.OUT_FIELDS for the real particles, the dt scalars, and the gather and
+symmetric neighbour sets (i, j_real, image code; 0-based).
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, HERE)
+
+from ndspmhd_amd.setups import periodic_mhd_box, dusty_box  # noqa: E402
+from refio import run_reference, SCALARS  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref", "ndsp_ref3d")
+
+
+def perturb_mhd(st, seed):
+    """Non-uniform B, psi, switches and energies so every MHD term is live."""
+    rng = np.random.default_rng(seed)
+    n = st["pmass"].shape[0]
+    st["Bevol"] += 0.05 * rng.standard_normal((n, 3))
+    st["psi"][:] = 0.01 * rng.standard_normal(n)
+    st["alpha"][:, 0] = rng.uniform(0.1, 1.0, n)
+    st["alpha"][:, 1] = rng.uniform(0.0, 1.0, n)
+    st["en"] *= rng.uniform(0.8, 1.2, n)
+    st["pmass"] *= rng.uniform(0.9, 1.1, n)
+    return st
+
+
+def cases():
+    o, st = periodic_mhd_box(8, seed=11, vel_seed=12)
+    st = perturb_mhd(st, 13)
+    yield "mhd3d_nr", o, st
+    yield "mhd3d_tight", o.replace(tolh=1.0e-6), st
+    yield "mhd3d_fixedh", o.replace(maxdensits=0), st
+    o, st = periodic_mhd_box(10, seed=21, vel_seed=22, jitter=0.25)
+    st = perturb_mhd(st, 23)
+    o = o.replace(iavlim=(2, 1, 1), xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5))
+    st["x"] -= 0.5
+    yield "mhd3d_resist", o, st
+    o, st = dusty_box(8, ndim=3, seed=31)
+    rng = np.random.default_rng(32)
+    st["vel"][:] = 0.05 * rng.standard_normal(st["vel"].shape)
+    yield "dust3d_nr", o, st
+    yield "dust3d_fixedh", o.replace(maxdensits=0), st
+
+
+def main():
+    if not os.path.exists(REF):
+        raise SystemExit("build the reference first: make -C oracle/ref")
+    for name, o, st in cases():
+        out = run_reference(REF, o, st, mode=2)
+        rec = {"in_iopt": o.iopt(), "in_dopt": o.dopt()}
+        for k, v in st.items():
+            rec["in_" + k] = v
+        for k, v in out.items():
+            if k in ("log",):
+                continue
+            if k in ("pairs_gather", "pairs_sym"):
+                v = v.copy()
+                v[:, :2] -= 1
+                rec["out_" + k] = v.astype(np.int32)
+            elif isinstance(v, np.ndarray):
+                rec["out_" + k] = v
+            else:
+                rec["out_" + k] = np.asarray(v)
+        path = os.path.join(HERE, name + ".npz")
+        np.savez_compressed(path, **rec)
+        print("%-14s N=%d its=%d ntotal=%d  -> %s (%d KB)" % (
+            name, out["npart"], out["itsdensity"], out["ntotal"], os.path.basename(path),
+            os.path.getsize(path) // 1024))
+
+
+if __name__ == "__main__":
+    main()
