@@ -1,0 +1,198 @@
+#!/usr/bin/env python3
+"""Throughput of the ndspmhd hot path on one MI355X.
+
+Workload: 3D periodic unit box, nside^3 particles (default 128^3) on a cubic
+lattice with uniform random offsets of up to 0.1 spacing, rho = 1, Gaussian
+random velocity field P(k) ~ k^-4 at rms Mach 1, uniform B = (0.1, 0, 0),
+gamma = 5/3, cubic spline, hfact = 1.2 (SPMHD with hyperbolic divB cleaning,
+artificial viscosity/conductivity/resistivity with switches).  One step =
+one leapfrog step (predictor, derivs = binning + h-rho iteration + fused
+force sums, corrector, new dt), all on the device with the state resident in
+HBM.  Synthetic data.
+
+Metric: pair interactions/s = (density pairs summed over h-rho passes +
+unordered force pairs) / wall time, plus steps/s; the same work in plain FP64
+on the GPU and the compiled reference (serial FP64 CPU) are timed beside it.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic compulsory bytes per particle and launch, reference layout
+# (8 B per real, each field read once and each output written once)
+BYTES_FORCE = 280        # 20 reals in, 15 reals out
+BYTES_DENSITY = 108      # per active particle and h-rho pass
+FLOPS_FORCE_PAIR = 200   # FP32 flops per ordered pair evaluation (estimate, see DESIGN)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_gpu(nside, steps, warmup, prec, timing=True):
+    import ndspmhd_amd as nd
+    from ndspmhd_amd.setups import periodic_mhd_box
+    o, st = periodic_mhd_box(nside)
+    sim = nd.Simulation(o, st, precision=prec)
+    sim.derivs()
+    for _ in range(warmup):
+        sim.step()
+    sim.set_timing(timing)
+    pd = pf = 0
+    nits = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step()
+        pd += sim.stats["pairs_density"]
+        pf += sim.stats["pairs_force"]
+        nits.append(sim.stats["ncalc_mean"])
+    t1 = time.perf_counter()
+    kt = sim.kernel_times() if timing else {}
+    return dict(sim=sim, opts=o, wall=t1 - t0, pairs_density=pd, pairs_force=pf,
+                ncalc_mean=float(np.mean(nits)), kernel_ms=kt, n=nside ** 3)
+
+
+def field_errors(nside):
+    """mixed vs FP64 GPU on one derivs of the same state (norm-relative L-inf)"""
+    import ndspmhd_amd as nd
+    from ndspmhd_amd.setups import periodic_mhd_box
+    o, st = periodic_mhd_box(nside)
+    o = o.replace(maxdensits=0)
+    a = nd.Simulation(o, st, precision=0)
+    b = nd.Simulation(o, st, precision=1)
+    a.derivs()
+    b.derivs()
+    out = {}
+    for k in ("rho", "force", "dBevoldt", "dudt", "divB"):
+        x, y = a.get(k), b.get(k)
+        out[k] = float(np.max(np.abs(x - y)) / max(np.max(np.abs(y)), 1e-300))
+    a.close()
+    b.close()
+    return out
+
+
+def cpu_baseline(sample_nside):
+    """The compiled reference (serial FP64 Fortran, oracle/_ref) on a bounded
+    sample of the same workload; pair counts of the sample come from the GPU
+    run of the identical input (the count is a property of the input)."""
+    import ndspmhd_amd as nd
+    from ndspmhd_amd.setups import periodic_mhd_box
+    o, st = periodic_mhd_box(sample_nside)
+    g = nd.Simulation(o, st, precision=1)
+    g.derivs()
+    pairs = g.stats["pairs_density"] + g.stats["pairs_force"]
+    g.close()
+    ref = os.path.join(ROOT, "oracle", "_ref", "ndsp_ref3d")
+    if os.path.exists(ref):
+        sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+        from refio import run_reference
+        out = run_reference(ref, o, st, mode=1, timeout=900)
+        secs = [float(l.split()[1]) for l in out["log"].splitlines() if "NDSP_DERIVS_SECONDS" in l]
+        t = secs[0]
+        kind = "reference"
+        what = "compiled reference derivs (amdflang -O2, serial)"
+    else:
+        import oracle
+        orc = oracle.Oracle(o, st)
+        t0 = time.perf_counter()
+        orc.derivs()
+        t = time.perf_counter() - t0
+        kind = "port"
+        what = "CPU oracle derivs (C restatement, serial)"
+    return dict(value=pairs / t, unit="pair interactions/s", cores=1, kind=kind,
+                sample="%s on the %d^3 box (%d particles, %d pairs, %.2f s)" % (
+                    what, sample_nside, sample_nside ** 3, pairs, t))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nside", type=int, default=128)
+    ap.add_argument("--fp64-steps", type=int, default=5)
+    ap.add_argument("--cpu-nside", type=int, default=40)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU sharding is "
+              "design-only (see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    log("[bench] mixed precision, %d^3, warmup %d, steps %d" % (args.nside, args.warmup, args.steps))
+    r = run_gpu(args.nside, args.steps, args.warmup, prec=0)
+    n = r["n"]
+    pairs = r["pairs_density"] + r["pairs_force"]
+    value = pairs / r["wall"]
+    ms_step = 1e3 * r["wall"] / args.steps
+    kt = r["kernel_ms"]
+    log("[bench] %.3f ms/step, %.3e pairs/s, kernels %s" % (ms_step, value, kt))
+    dom = max(kt, key=kt.get) if kt else "force"
+    force_ms = kt.get("force", float("nan"))
+    dens_ms = kt.get("density", float("nan"))
+    # algorithmic bytes per launch of the dominant kernel
+    if dom == "density":
+        alg_bytes = BYTES_DENSITY * n * r["ncalc_mean"]
+    else:
+        dom = "force"
+        alg_bytes = BYTES_FORCE * n
+    dom_ms = kt.get(dom, float("nan"))
+    achieved = alg_bytes / (dom_ms * 1e-3) / 1e9
+    pairs_force_step = r["pairs_force"] / args.steps
+    flops = FLOPS_FORCE_PAIR * 2 * pairs_force_step
+    result = {
+        "metric": "pair interactions/s (3D 128^3 SPMHD leapfrog step, density + force pairs)",
+        "value": value,
+        "unit": "pair interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "steps_per_s": 1e3 / ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair sums (compensated), f64 positions/integrator/reductions",
+        "data": "synthetic",
+        "config": {"workload": "3D periodic MHD box %d^3 (%d particles), cubic spline, hfact 1.2, "
+                               "gamma 5/3, v_rms Mach 1 P(k)~k^-4, B=(0.1,0,0)" % (args.nside, n),
+                   "particles": n, "steps": args.steps},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": dom_ms,
+                     "force_fp32_tflops": flops / (force_ms * 1e-3) / 1e12 if force_ms == force_ms else None},
+        "kernel_ms": kt,
+        "pairs_per_step": {"density": r["pairs_density"] / args.steps, "force": pairs_force_step},
+        "ncalc_mean": r["ncalc_mean"],
+    }
+    r["sim"].close()
+    if not args.no_fp64:
+        log("[bench] plain FP64 GPU, %d steps" % args.fp64_steps)
+        r64 = run_gpu(args.nside, args.fp64_steps, 1, prec=1, timing=False)
+        v64 = (r64["pairs_density"] + r64["pairs_force"]) / r64["wall"]
+        result["fp64_gpu"] = {"value": v64, "ms_per_step": 1e3 * r64["wall"] / args.fp64_steps,
+                              "speedup_mixed_over_fp64": value / v64}
+        r64["sim"].close()
+        result["error_vs_fp64"] = field_errors(min(args.nside, 64))
+    if not args.no_cpu:
+        log("[bench] CPU baseline on %d^3" % args.cpu_nside)
+        result["cpu_baseline"] = cpu_baseline(args.cpu_nside)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

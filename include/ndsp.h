@@ -1,0 +1,156 @@
+/*
+ * ndsp.h -- C-ABI of the MI355X hot-path library (libndsp_hip.so).
+ *
+ * The reference (ndspmhd, serial Fortran) has no plugin or FFI layer: its hot
+ * path is a set of argument-less subroutines that share module globals and
+ * are swapped at link time (src/Makefile:227,263).  This header is the
+ * boundary a Fortran host binds with iso_c_binding (see
+ * ndspmhd_amd/fortran/ndsp_host.f90): each entry point below replaces one
+ * reference routine, cited beside it.  Plain pointers and sizes only; every
+ * entry returns an ndsp_status (0 = ok), the text of the last error is
+ * ndsp_last_error().  One context = one simulation resident in HBM of the
+ * current HIP device; a context is not thread-safe.
+ *
+ * Option vectors (same slots as the reference harness and the oracle):
+ *   iopt[32]: 0 ikernel, 1 iener, 2 imhd, 3 imagforce, 4 idivbzero, 5 iav,
+ *             6-8 iavlim(1:3), 9 idust, 10 idrag_nature, 11-13 ibound(1:3),
+ *             14 maxdensits, 15 idustevol, 16 ndim
+ *   dopt[32]: 0 hfact, 1 tolh, 2 gamma, 3 polyk, 4 Kdrag, 5 alphamin,
+ *             6 alphaumin, 7 alphaBmin, 8 beta, 9 avdecayconst,
+ *             10 psidecayfact, 11 C_cour, 12 C_force, 13-15 xmin(1:3),
+ *             16-18 xmax(1:3), 19 psep, 20-22 Bconst(1:3), 23 dt, 24 damp
+ *   (src/variablesND.f90:33-300 module globals; defaults src/defaults.f90)
+ * Particle arrays use the reference's column-major layout
+ * (x(ndim,N) -> x[i*ndim+d], vel(3,N) -> vel[i*3+d]; src/variablesND.f90:161-188).
+ */
+#ifndef NDSP_H
+#define NDSP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ndsp_ctx ndsp_ctx;
+
+typedef enum {
+  NDSP_OK = 0,
+  NDSP_ERR_H_NONPOSITIVE = 1,    /* h <= 0 (src/iterate_density.f90:99-102) */
+  NDSP_ERR_RHO_NONPOSITIVE = 2,  /* rho <= 0 (src/iterate_density.f90:168-170) */
+  NDSP_ERR_NOT_CONVERGED = 3,    /* h-rho iteration (src/iterate_density.f90:349-351) */
+  NDSP_ERR_EOS = 4,              /* u < 0 (src/eos.f90:64-72) */
+  NDSP_ERR_CELLS = 5,            /* cell grid impossible (src/linkND.f90:73-101) */
+  NDSP_ERR_OUTSIDE = 6,          /* particle outside domain (src/linkND.f90:122-125) */
+  NDSP_ERR_ALLOC = 7,
+  NDSP_ERR_COINCIDENT = 8,       /* dx = 0 pair (src/ratesND_mhd.f90:384-395) */
+  NDSP_ERR_VSIG = 9,             /* vsig determinant < 0 (src/ratesND_mhd.f90:1422-1429) */
+  NDSP_ERR_BAD_OPTION = 10,      /* option combination not implemented */
+  NDSP_ERR_BAD_ARG = 11,
+  NDSP_ERR_HIP = 12,             /* HIP runtime error */
+  NDSP_ERR_NO_DEVICE = 13
+} ndsp_status;
+
+/* arithmetic of the pair sums */
+enum {
+  NDSP_PREC_MIXED = 0, /* FP64 positions/neighbour tests/h-rho update/integrator/
+                          reductions, FP32 pair math with compensated sums */
+  NDSP_PREC_FP64 = 1   /* everything FP64, plain sums */
+};
+
+typedef struct {
+  int32_t itsdensity;   /* max Newton-Raphson passes over particles (itsdensity) */
+  int32_t nmin, nmax;   /* min/max neighbour counts (numneigh) */
+  int32_t pad;
+  double ncalc_mean;    /* mean passes per particle (ncalctotal/npart) */
+  int64_t pairs_density;/* ordered gather pairs summed over all passes */
+  int64_t pairs_force;  /* unordered symmetric pairs in the force pass */
+} ndsp_stats;
+
+typedef struct {
+  double dt;            /* next timestep (src/stepND_leapfrog_mhd.f90:252) */
+  double dtcourant, dtforce, dtdrag, vsig2max, hhmax, stressmax, time;
+} ndsp_dt;
+
+/* Context with capacity for npart particles; replaces the module-global
+ * set-up of initialise (src/initialiseND_mhd.f90:140-300: kernel tables via
+ * setkernels src/kernelND.f90:102, avfact :166-172) and alloc
+ * (src/allocateND.f90:34).  precision: NDSP_PREC_*. */
+int ndsp_create(const int32_t *iopt, const double *dopt, int64_t npart,
+                int32_t precision, ndsp_ctx **out);
+int ndsp_destroy(ndsp_ctx *ctx);
+
+/* Host -> HBM copy of the evolved state (reference layout); itype may be
+ * NULL (all gas).  Replaces nothing in the reference (its state lives in
+ * module arrays); the Fortran host calls it after setup and after any host
+ * edit of the arrays. */
+int ndsp_upload(ndsp_ctx *ctx, const double *x, const double *vel,
+                const double *pmass, const double *hh, const double *en,
+                const double *Bevol, const double *alpha, const double *psi,
+                const double *dustevol, const int32_t *itype);
+
+/* HBM -> host copy of one per-particle field of the real particles, in the
+ * caller's particle order and reference layout.  Names as the reference's
+ * arrays: x vel pmass hh en Bevol alpha psi dustevol rho gradh gradgradh
+ * drhodt dhdt dens pr spsound uu Bfield force dudt dendt dBevoldt daldt
+ * dpsidt divB curlB gradpsi dustfrac rhogas rhodust ddustevoldt deltav
+ * numneigh nits.  *width receives 1, 3 or ndim. */
+int ndsp_download(ndsp_ctx *ctx, const char *name, double *out, int32_t *width);
+
+/* Replaces set_linklist (src/linkND.f90:45) together with the ghost set-up
+ * it relies on (src/ghostND_mhd.f90:33, src/boundaryND.f90:30): periodic
+ * wrap / reflection, then binning into cells of side >= radkern*hmax by a
+ * counting sort on cell keys. */
+int ndsp_set_linklist(ndsp_ctx *ctx);
+
+/* Replaces iterate_density (src/iterate_density.f90:43) with density /
+ * density_partial (src/density_sums.f90:38,396): per-particle h-rho
+ * Newton-Raphson to tolh, then conservative2primitive
+ * (src/conservative2primitive.f90:42) and the equation of state
+ * (src/eos.f90:40).  Needs ndsp_set_linklist first. */
+int ndsp_iterate_density(ndsp_ctx *ctx, ndsp_stats *stats);
+
+/* Replaces get_rates (src/ratesND_mhd.f90:29): fused pair sums and
+ * per-particle finalisation; fills the timestep criteria. */
+int ndsp_get_rates(ndsp_ctx *ctx, ndsp_dt *dt);
+
+/* Replaces derivs (src/derivs.f90:27): set_linklist, iterate_density,
+ * get_rates in order. */
+int ndsp_derivs(ndsp_ctx *ctx, ndsp_stats *stats, ndsp_dt *dt);
+
+/* Replaces step (src/stepND_leapfrog_mhd.f90:31): predictor, derivs,
+ * corrector, boundary, new dt, all on the device.  Uses the context's dt
+ * (dopt[23] at creation, then the value the previous step chose). */
+int ndsp_step(ndsp_ctx *ctx, ndsp_stats *stats, ndsp_dt *dt);
+
+/* Sets the timestep and time (evolve.f90:52-100 keeps both in module
+ * timestep). */
+int ndsp_set_dt(ndsp_ctx *ctx, double dt, double time);
+
+/* FP64 conserved quantities of the real particles (src/evwrite_mhd.f90:124-248):
+ * out[17] = ekin etherm emag etot momx momy momz momtot angx angy angz angtot
+ * rhomin rhomax rhomean totmassgas totmassdust. */
+int ndsp_conserved(ndsp_ctx *ctx, double *out);
+
+/* Neighbour sets at the current h (test support; the reference's
+ * equivalent is the brute-force check_neighbourlist,
+ * src/check_neighbourlist.f90:27): gather (q2_i < R^2) if sym=0, symmetric
+ * (q2_i < R^2 or q2_j < R^2) if sym=1.  pairs[3*k..] = (i, j, image code),
+ * caller's 0-based indices; code digits per dimension (x least significant,
+ * base 3): 0 none, 1 image at larger coordinate, 2 at smaller.  Returns
+ * NDSP_ERR_BAD_ARG with *npairs = needed count when cap is too small. */
+int ndsp_neighbours(ndsp_ctx *ctx, int32_t sym, int32_t *pairs, int64_t cap,
+                    int64_t *npairs);
+
+/* Average duration (ms) of each kernel of the last ndsp_step / ndsp_derivs,
+ * measured with HIP events on the context's stream; names[] receives
+ * comma-separated kernel labels.  Timing is off unless enabled. */
+int ndsp_set_timing(ndsp_ctx *ctx, int32_t on);
+int ndsp_kernel_times(ndsp_ctx *ctx, double *ms, int32_t cap, int32_t *n,
+                      char *names, int32_t names_cap);
+
+const char *ndsp_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

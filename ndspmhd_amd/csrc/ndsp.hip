@@ -1,0 +1,1980 @@
+// MI355X (gfx950) implementation of the ndspmhd hot path behind the C-ABI in
+// include/ndsp.h.  One context holds one simulation resident in HBM.
+//
+// Pipeline of one derivative evaluation (src/derivs.f90:27):
+//   k_boundary      periodic wrap / wall reflection (src/boundaryND.f90:66-160)
+//   k_binstats      max h and bounding box (src/ghostND_mhd.f90:77, linkND.f90:78-90)
+//   k_cellid, scan, k_scatter, k_cellsort
+//                   counting sort of particles by linear cell key (x fastest),
+//                   stable by slot: replaces the head/next link lists of
+//                   set_linklist (src/linkND.f90:45-161)
+//   k_pack_density  sorted neighbour view: FP64 (x,y,z,1/h) + payload
+//   k_nr_pass       h-rho Newton-Raphson passes, persistent once a pass is partial
+//                   (src/iterate_density.f90:43-360 with density_sums.f90),
+//                   fused conservative->primitive + EOS
+//                   (src/conservative2primitive.f90:42-470, src/eos.f90:40-106)
+//   k_pack_force    sorted neighbour view for the force pass
+//   k_force         fused pair sums of get_rates (src/ratesND_mhd.f90:304-467,
+//                   1175-1690, 1700-1894, 2154-2206, 2377-2720, 2813-2878)
+//   k_finalize      per-particle finalisation (src/ratesND_mhd.f90:482-943)
+//   k_dt            timestep (src/stepND_leapfrog_mhd.f90:252)
+// and of one step: k_predict, derivs, k_correct (stepND_leapfrog_mhd.f90:31-275).
+//
+// Periodic and reflecting boundaries use image cells instead of ghost copies;
+// the image coordinates reproduce the ghost arithmetic of
+// src/ghostND_mhd.f90:180-200 bit for bit, so neighbour sets are identical.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ndsp.h"
+#include "ndsp_kernels.h"
+
+static thread_local std::string g_err;
+
+static int set_err(int code, const std::string& m) {
+  g_err = m;
+  return code;
+}
+
+#define HIPCHK(call)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return set_err(NDSP_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+static const int BS = 256;
+
+// ===========================================================================
+// small kernels: boundary, statistics, binning, scan
+// ===========================================================================
+
+// boundary, src/boundaryND.f90:66-160
+__global__ void k_boundary(KArgs a, Phys p, Grid g, int in_step) {
+#pragma clang fp contract(off)
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  double* xs[3] = {a.f[F_X], a.f[F_Y], a.f[F_Z]};
+  double* vs[3] = {a.f[F_VX], a.f[F_VY], a.f[F_VZ]};
+  double* xins[3] = {a.f[F_XIN], a.f[F_YIN], a.f[F_ZIN]};
+  double* vins[3] = {a.f[F_VXIN], a.f[F_VYIN], a.f[F_VZIN]};
+  double* fs[3] = {a.f[F_FX], a.f[F_FY], a.f[F_FZ]};
+  for (int d = 0; d < p.ndim; d++) {
+    if (g.bound[d] != 3) continue;
+    double x = xs[d][i];
+    if (x > g.xmax[d]) x = g.xmin[d] + x - g.xmax[d];
+    else if (x < g.xmin[d]) x = g.xmax[d] - (g.xmin[d] - x);
+    xs[d][i] = x;
+  }
+  for (int d = 0; d < p.ndim; d++) {
+    if (g.bound[d] <= 1) continue;
+    double x = xs[d][i];
+    bool hit = false;
+    if (x > g.xmax[d]) { x = g.xmax[d] - (x - g.xmax[d]); hit = true; }
+    else if (x < g.xmin[d]) { x = g.xmin[d] + (g.xmin[d] - x); hit = true; }
+    if (hit) {
+      xs[d][i] = x;
+      vs[d][i] = -vs[d][i];
+      if (in_step) { xins[d][i] = x; vins[d][i] = vs[d][i]; }
+      fs[d][i] = 0.0;
+    }
+  }
+}
+
+// per-block max h and bounding box -> part[blk*8 + 0..6]
+__global__ void k_binstats(KArgs a, int ndim, double* part) {
+  __shared__ double sm[7][BS];
+  int t = threadIdx.x;
+  double hm = 0.0, lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+  const double* xs[3] = {a.f[F_X], a.f[F_Y], a.f[F_Z]};
+  for (int i = blockIdx.x * blockDim.x + t; i < a.n; i += gridDim.x * blockDim.x) {
+    hm = fmax(hm, a.f[F_H][i]);
+    for (int d = 0; d < ndim; d++) {
+      lo[d] = fmin(lo[d], xs[d][i]);
+      hi[d] = fmax(hi[d], xs[d][i]);
+    }
+  }
+  sm[0][t] = hm;
+  for (int d = 0; d < 3; d++) { sm[1 + d][t] = lo[d]; sm[4 + d][t] = hi[d]; }
+  __syncthreads();
+  for (int s = BS / 2; s > 0; s >>= 1) {
+    if (t < s) {
+      sm[0][t] = fmax(sm[0][t], sm[0][t + s]);
+      for (int d = 0; d < 3; d++) {
+        sm[1 + d][t] = fmin(sm[1 + d][t], sm[1 + d][t + s]);
+        sm[4 + d][t] = fmax(sm[4 + d][t], sm[4 + d][t + s]);
+      }
+    }
+    __syncthreads();
+  }
+  if (t < 7) part[blockIdx.x * 8 + t] = sm[t][0];
+}
+
+__global__ void k_binstats_final(DScal* sc, const double* part, int nb) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double hm = 0.0, lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+  for (int b = 0; b < nb; b++) {
+    hm = fmax(hm, part[b * 8]);
+    for (int d = 0; d < 3; d++) {
+      lo[d] = fmin(lo[d], part[b * 8 + 1 + d]);
+      hi[d] = fmax(hi[d], part[b * 8 + 4 + d]);
+    }
+  }
+  sc->hmax_bin = hm;
+  for (int d = 0; d < 3; d++) { sc->bbox_lo[d] = lo[d]; sc->bbox_hi[d] = hi[d]; }
+}
+
+__device__ __forceinline__ int cell_coord(const Grid& g, int d, double x) {
+  int c = (int)floor((x - g.lo[d]) * g.inv_w[d]);
+  if (c < 0) c = 0;
+  if (c > g.nc[d] - 1) c = g.nc[d] - 1;
+  return c;
+}
+
+__global__ void k_cellid(KArgs a, Grid g) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  int cx = cell_coord(g, 0, a.f[F_X][i]);
+  int cy = g.ndim >= 2 ? cell_coord(g, 1, a.f[F_Y][i]) : 0;
+  int cz = g.ndim >= 3 ? cell_coord(g, 2, a.f[F_Z][i]) : 0;
+  int c = cx + g.nc[0] * (cy + g.nc[1] * cz);
+  a.cellid[i] = c;
+  atomicAdd(&a.cfill[c], 1);
+}
+
+// exclusive scan, 3 phases (1024 elements per block)
+__global__ void k_scan_local(const int* in, int* out, int* bsum, long long n) {
+  __shared__ int sm[BS];
+  const int ITEMS = 4;
+  long long base = (long long)blockIdx.x * BS * ITEMS + (long long)threadIdx.x * ITEMS;
+  int v[ITEMS];
+  int tot = 0;
+  for (int k = 0; k < ITEMS; k++) {
+    long long idx = base + k;
+    v[k] = idx < n ? in[idx] : 0;
+    tot += v[k];
+  }
+  sm[threadIdx.x] = tot;
+  __syncthreads();
+  for (int s = 1; s < BS; s <<= 1) {
+    int x = threadIdx.x >= s ? sm[threadIdx.x - s] : 0;
+    __syncthreads();
+    sm[threadIdx.x] += x;
+    __syncthreads();
+  }
+  int run = sm[threadIdx.x] - tot;
+  for (int k = 0; k < ITEMS; k++) {
+    long long idx = base + k;
+    if (idx < n) out[idx] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == BS - 1) bsum[blockIdx.x] = sm[BS - 1];
+}
+
+__global__ void k_scan_bsum(int* bsum, int nb, int* total_out) {
+  __shared__ int sm[BS];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nb; base += BS) {
+    int idx = base + threadIdx.x;
+    int v = idx < nb ? bsum[idx] : 0;
+    sm[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 1; s < BS; s <<= 1) {
+      int x = threadIdx.x >= s ? sm[threadIdx.x - s] : 0;
+      __syncthreads();
+      sm[threadIdx.x] += x;
+      __syncthreads();
+    }
+    if (idx < nb) bsum[idx] = carry + sm[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == BS - 1) carry += sm[BS - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+__global__ void k_scan_add(int* out, const int* bsum, long long n) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < n) out[idx] += bsum[idx / (BS * 4)];
+}
+
+__global__ void k_scatter(KArgs a) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  int c = a.cellid[i];
+  int pos = a.cstart[c] + atomicAdd(&a.cfill[c], 1);
+  a.sidx[pos] = i;
+}
+
+// make the order inside each cell ascending in slot index (deterministic,
+// and a stable sort by cell key overall)
+__global__ void k_cellsort(KArgs a, long long ncell) {
+  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  int b = a.cstart[c], e = a.cstart[c + 1];
+  for (int k = b + 1; k < e; k++) {
+    int v = a.sidx[k];
+    int j = k - 1;
+    while (j >= b && a.sidx[j] > v) {
+      a.sidx[j + 1] = a.sidx[j];
+      j--;
+    }
+    a.sidx[j + 1] = v;
+  }
+}
+
+// ===========================================================================
+// neighbour views
+// ===========================================================================
+
+template <typename PT>
+__global__ void k_pack_density(KArgs a, Phys p) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.n) return;
+  int i = a.sidx[k];
+  double h = a.f[F_H][i];
+  a.P4[k] = make_double4(a.f[F_X][i], a.f[F_Y][i], a.f[F_Z][i], 1.0 / h);
+  a.dxb[k] = p.radkern * h;
+  typedef typename V4<PT>::type T4;
+  ((T4*)a.D4)[k] = T4{(PT)a.f[F_VX][i], (PT)a.f[F_VY][i], (PT)a.f[F_VZ][i], (PT)a.f[F_M][i]};
+  // the density sums read the dust fraction of the last conservative2primitive
+  // (dustfrac, src/density_sums.f90:276-278), not the predicted dustevol
+  if (p.idust) ((PT*)a.De)[k] = (PT)a.f[F_DUSTFRAC][i];
+}
+
+template <typename PT>
+__global__ void k_pack_force(KArgs a, Phys p) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.n) return;
+  int i = a.sidx[k];
+  double h = a.f[F_H][i];
+  a.P4[k].w = 1.0 / h;
+  typedef typename V4<PT>::type T4;
+  ((T4*)a.G0)[k] = T4{(PT)a.f[F_VX][i], (PT)a.f[F_VY][i], (PT)a.f[F_VZ][i], (PT)a.f[F_M][i]};
+  ((T4*)a.G1)[k] = T4{(PT)a.f[F_RHO][i], (PT)a.f[F_PR][i], (PT)a.f[F_CS][i], (PT)a.f[F_GRADH][i]};
+  ((T4*)a.G2)[k] = T4{(PT)a.f[F_UU][i], (PT)a.f[F_A0][i], (PT)a.f[F_A1][i], (PT)a.f[F_A2][i]};
+  if (p.imhd)
+    ((T4*)a.G3)[k] = T4{(PT)a.f[F_BX][i], (PT)a.f[F_BY][i], (PT)a.f[F_BZ][i], (PT)a.f[F_PSI][i]};
+  ((T4*)a.G4)[k] = T4{(PT)h, (PT)a.f[F_DENS][i], (PT)a.f[F_DUSTFRAC][i], (PT)a.f[F_RHOGAS][i]};
+  if (p.idust) ((PT*)a.G5)[k] = (PT)a.f[F_RHODUST][i];
+}
+
+// ===========================================================================
+// neighbour iteration over image cells
+// ===========================================================================
+
+// Calls body(kk, xj_img[3], flip[3], is_image) for every particle of every
+// cell image overlapping [xi - rad, xi + rad]; xj_img holds the image
+// coordinates (ghost arithmetic of src/ghostND_mhd.f90:180-200).
+template <int NDIM, typename F>
+__device__ __forceinline__ void for_each_candidate(const KArgs& a, const Grid& g, const double* xi,
+                                                   double rad, F&& body) {
+  int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  for (int d = 0; d < NDIM; d++) cell_range(g, d, xi[d], rad, lo[d], hi[d]);
+  for (int cz = lo[2]; cz <= hi[2]; cz++) {
+    int kz = 0, sz = 0;
+    if (NDIM >= 3) { sz = cell_image(g, 2, cz, kz); if (sz < 0) continue; }
+    for (int cy = lo[1]; cy <= hi[1]; cy++) {
+      int ky = 0, sy = 0;
+      if (NDIM >= 2) { sy = cell_image(g, 1, cy, ky); if (sy < 0) continue; }
+      const int rowbase = g.nc[0] * (sy + g.nc[1] * sz);
+      if (lo[0] >= 0 && hi[0] < g.nc[0] && kz == 0 && ky == 0) {
+        // contiguous run of cells with no image: one particle range
+        const int b = a.cstart[rowbase + lo[0]], e = a.cstart[rowbase + hi[0] + 1];
+        for (int kk = b; kk < e; kk++) {
+          double4 pj = a.P4[kk];
+          double xj[3] = {pj.x, pj.y, pj.z};
+          const int flip[3] = {0, 0, 0};
+          body(kk, xj, flip, false);
+        }
+        continue;
+      }
+      for (int cx = lo[0]; cx <= hi[0]; cx++) {
+        int kx = 0;
+        int sx = cell_image(g, 0, cx, kx);
+        if (sx < 0) continue;
+        const int kind[3] = {kx, ky, kz};
+        const bool img = (kx | ky | kz) != 0;
+        const int b = a.cstart[rowbase + sx], e = a.cstart[rowbase + sx + 1];
+        for (int kk = b; kk < e; kk++) {
+          double4 pj = a.P4[kk];
+          double xj[3] = {pj.x, pj.y, pj.z};
+          int flip[3] = {0, 0, 0};
+          bool ok = true;
+          if (img) {
+            for (int d = 0; d < NDIM; d++) {
+              if (!kind[d]) continue;
+              ok = ok && image_coord(g, d, kind[d], xj[d], a.dxb[kk], xj[d]);
+              flip[d] = kind[d] >= 3;
+            }
+          }
+          if (ok) body(kk, xj, flip, img);
+        }
+      }
+    }
+  }
+}
+
+// ===========================================================================
+// density: h-rho Newton-Raphson, src/iterate_density.f90:43-360
+//
+// One launch runs pass `its0` for the listed particles (all when list is
+// null).  sym = 1 is a pass of the symmetric routine density
+// (src/density_sums.f90:38), which the reference uses as long as every
+// particle is still iterating (ncalc = npart, iterate_density.f90:131-135):
+// the neighbour count is over the symmetric set and needs each neighbour's
+// current h (P4.w), and psi is left alone unless MHD is off.  sym = 0 is
+// density_partial (src/density_sums.f90:396): gather count, psi zeroed when
+// MHD is on (:630-635).  With persistent = 1 each particle keeps iterating
+// (partial passes) until it converges; otherwise an unconverged particle is
+// appended to next_list for the next launch.  Sums depend only on the
+// particle's own h, so the per-particle sequence of h is the reference's.
+// ===========================================================================
+
+
+// Newton-Raphson step for h of one particle from its sums at the current h
+// (src/iterate_density.f90:162-278; FP64, reference operation order)
+struct NRStep {
+  double hnew, gradh, ggh, dhdrho;
+  bool converged, relink;
+};
+
+template <int NDIM>
+__device__ __forceinline__ void nr_update(const Phys& p, double hh, double hin, double mi, double rho_v,
+                                          double gradh_sum, double ggh_sum, int nn, int its, NRStep& r) {
+#pragma clang fp contract(off)
+  const double t = hh / p.hfact;
+  const double rhoi = mi / (NDIM == 1 ? t : NDIM == 2 ? t * t : t * t * t);
+  const double dhdrhoi = -hh / (NDIM * rho_v);
+  const double dwdhsumi = gradh_sum;
+  double omegai = 1.0 - dhdrhoi * gradh_sum;
+  if (omegai < 1.e-5 && fabs(omegai) == 0.0) omegai = 1.0;
+  r.gradh = 1.0 / omegai;
+  const double func = rhoi - rho_v;
+  const double dfdh = omegai / dhdrhoi;
+  const double rn = rho_v * NDIM;
+  const double d2hdrho2i = hh * (NDIM + 1) / (rn * rn);
+  r.ggh = rho_v * (d2hdrho2i * dwdhsumi + dhdrhoi * dhdrhoi * ggh_sum);
+  double hnew = hh - func / dfdh;
+  if (hnew > 1.2 * hh) hnew = 1.2 * hh;
+  else if (hnew < 0.8 * hh) hnew = 0.8 * hh;
+  if (hnew <= 0.0 || r.gradh <= DBL_MIN) hnew = p.hfact * pow(mi / rho_v, p.dndim);
+  else if (its > 100) hnew = p.hfact * pow(mi / rho_v, p.dndim);
+  r.relink = false;
+  if (nn <= 1) { hnew = hh + p.psep; r.relink = true; }
+  r.converged = (fabs((hnew - hh) / hin) < p.tolh && omegai > 0.0) || p.maxits == 0;
+  r.hnew = hnew;
+  r.dhdrho = dhdrhoi;
+}
+
+template <int NDIM, typename PT>
+__global__ __launch_bounds__(BS) void k_nr_pass(KArgs a, Phys p, Grid g, const int* __restrict__ list, int nlist,
+                                                int its0, int sym, int persistent, int* __restrict__ next_list) {
+  __shared__ PT tw[NDSP_IKERN + 1];
+  __shared__ PT tg[NDSP_IKERN + 1];
+  __shared__ PT tgg[NDSP_IKERN + 1];
+  {
+    const PT* src = sizeof(PT) == 4 ? (const PT*)a.tabf : (const PT*)a.tabd;
+    for (int t = threadIdx.x; t <= NDSP_IKERN; t += blockDim.x) {
+      tw[t] = src[t];
+      tg[t] = src[NDSP_IKERN + 1 + t];
+      tgg[t] = src[2 * (NDSP_IKERN + 1) + t];
+    }
+  }
+  __syncthreads();
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = tid < nlist;
+  bool unconverged = false;
+  const int k = active ? (list ? list[tid] : tid) : 0;
+  typedef typename V4<PT>::type T4;
+  const T4* D4 = (const T4*)a.D4;
+  const PT* De = (const PT*)a.De;
+  if (active) {
+    const int i = a.sidx[k];
+    const double4 pk = a.P4[k];
+    const double xi[3] = {pk.x, pk.y, pk.z};
+    double h = a.f[F_H][i];
+    double hin;
+    if (its0 == 1) { hin = h; a.f[F_HNR0][i] = h; }
+    else hin = a.f[F_HNR0][i];
+    const double mi = a.f[F_M][i];
+    const T4 dk = D4[k];
+    const PT vi[3] = {dk.x, dk.y, dk.z};
+    const PT ddq2 = (PT)p.ddq2table, dq2 = (PT)p.dq2table;
+    const double R2 = p.radkern2;
+    const double hmaxc = a.sc->hmax_cur;
+    const bool dust = p.idust != 0;
+
+    int its = its0 - 1, nn = 0;
+    unsigned long long npairs = 0;
+    double rho_v = 0, drhodt_v = 0, gradh_v = 0, ggh_v = 0, rhog_v = 0, rhod_v = 0, dhdt_v = 0;
+    int err = 0;
+    bool done = false;
+    while (true) {
+      its++;
+      const bool symp = sym && its == its0;
+      if (!symp && p.imhd != 0) a.f[F_PSI][i] = 0.0;
+      if (symp && p.imhd == 0) a.f[F_PSI][i] = 0.0;
+      const double hi1 = 1.0 / h;
+      const double hi21 = mul_exact(hi1, hi1);
+      const PT hi1p = (PT)hi1;
+      const PT hfacwabi = pown_t<NDIM>(hi1p);
+      const PT hfacgrkerni = hfacwabi * hi1p;
+      const PT hi1s = hi1p * hi1p;
+      Acc<PT> rho, drhodt, gradh, ggh, rhog, rhod;
+      rho.zero(); drhodt.zero(); gradh.zero(); ggh.zero(); rhog.zero(); rhod.zero();
+      int nsym = 0, ngat = 0;
+      const double rad = p.radkern * (symp ? fmax(h, hmaxc) : h);
+      if (!(rad < p.boxmin)) { err = NDSP_ERR_CELLS; done = true; break; }
+      for_each_candidate<NDIM>(a, g, xi, rad, [&](int kk, const double* xj, const int* flip, bool img) {
+        double dx[3] = {0.0, 0.0, 0.0};
+        for (int d = 0; d < NDIM; d++) dx[d] = xi[d] - xj[d];
+        const double rij2 = rij2_exact<NDIM>(dx);
+        const double q2i = mul_exact(rij2, hi21);
+        if (symp) {
+          const double hj1 = a.P4[kk].w;
+          const double q2j = mul_exact(mul_exact(rij2, hj1), hj1);
+          if (q2i < R2 || q2j < R2) nsym++;
+        }
+        if (!(q2i < R2)) return;
+        ngat++;
+        const PT rij = sqrt((PT)rij2);
+        const PT q2 = (PT)q2i;
+        const PT wabi = ktab(tw, q2, ddq2, dq2) * hfacwabi;
+        const PT grkerni = ktab(tg, q2, ddq2, dq2) * hfacgrkerni;
+        const PT grgrkerni = ktab(tgg, q2, ddq2, dq2) * hfacwabi * hi1s;
+        const PT dwdhi = -(rij * grkerni * hi1p) - PT(NDIM) * wabi * hi1p;
+        const PT dwdhdhi = PT(NDIM * (NDIM + 1)) * wabi * hi1s + PT(2 * (NDIM + 1)) * rij * hi1s * grkerni +
+                           rij * rij * hi1s * grgrkerni;
+        const T4 dj = D4[kk];
+        const PT mj = dj.w;
+        rho.add(mj * wabi);
+        gradh.add(mj * dwdhi);
+        ggh.add(mj * dwdhdhi);
+        if (dust) {
+          const PT epsj = De[kk];
+          rhod.add(mj * epsj * wabi);
+          rhog.add(mj * (PT(1) - epsj) * wabi);
+        }
+        if (kk != k || img) {
+          const PT rinv = PT(1) / (rij + (PT)DBL_EPSILON);
+          const PT vj[3] = {flip[0] ? -dj.x : dj.x, flip[1] ? -dj.y : dj.y, flip[2] ? -dj.z : dj.z};
+          PT dvdotr = PT(0);
+          for (int d = 0; d < NDIM; d++) dvdotr += (vi[d] - vj[d]) * ((PT)dx[d] * rinv);
+          drhodt.add(mj * dvdotr * grkerni);
+        }
+      });
+      npairs += (unsigned long long)ngat;
+      nn = symp ? nsym : ngat;
+      // Newton-Raphson update, src/iterate_density.f90:162-278 (FP64)
+      rho_v = rho.value();
+      drhodt_v = drhodt.value();
+      if (dust) { rhog_v = rhog.value(); rhod_v = rhod.value(); }
+      if (rho_v <= 0.0) { err = NDSP_ERR_RHO_NONPOSITIVE; done = true; break; }
+      NRStep r;
+      nr_update<NDIM>(p, h, hin, mi, rho_v, gradh.value(), ggh.value(), nn, its, r);
+      gradh_v = r.gradh;
+      ggh_v = r.ggh;
+      const double hnew = r.hnew;
+      bool relink = r.relink;
+      if (r.converged) {
+        drhodt_v = mul_exact(drhodt_v, gradh_v);
+        dhdt_v = mul_exact(r.dhdrho, drhodt_v);
+        done = true;
+        break;
+      }
+      if (its > p.maxits) { err = NDSP_ERR_NOT_CONVERGED; done = true; break; }
+      if (hnew > a.sc->hhmax_ghost) relink = true;
+      if (relink) atomicOr(&a.sc->relink, 1);
+      h = hnew;
+      if (!persistent) {
+        a.f[F_H][i] = h;
+        unconverged = true;
+        break;
+      }
+    }
+    if (done) {
+      // ---- results, then conservative->primitive and EOS for this particle ----
+      a.f[F_H][i] = h;
+      a.f[F_RHO][i] = rho_v;
+      a.f[F_GRADH][i] = gradh_v;
+      a.f[F_GRADGRADH][i] = ggh_v;
+      a.f[F_DRHODT][i] = drhodt_v;
+      a.f[F_DHDT][i] = dhdt_v;
+      a.numneigh[i] = nn;
+      a.nits[i] = its;
+      double dens;
+      if (dust) {
+        double eps = a.f[F_EPS][i];
+        if (eps > 1.0) { eps = 1.0; a.f[F_EPS][i] = 1.0; a.f[F_EPSIN][i] = 1.0; }
+        a.f[F_DUSTFRAC][i] = eps;
+        a.f[F_RHOGAS][i] = rhog_v;
+        a.f[F_RHODUST][i] = rhod_v;
+        dens = rho_v * (1.0 - eps);
+      } else {
+        dens = rho_v;
+      }
+      a.f[F_DENS][i] = dens;
+      double uu = a.f[F_EN][i];
+      double pr = 0.0, cs = 0.0;
+      const double gamma1 = p.gamma - 1.0;
+      if (p.iener == 0) {
+        if (dens > 0.0) {
+          pr = p.polyk * pow(dens, p.gamma);
+          cs = sqrt(p.gamma * pr / dens);
+          if (fabs(gamma1) > 1.e-3) uu = pr / (gamma1 * dens);
+        }
+      } else {
+        if (uu < 0.0) err = err ? err : NDSP_ERR_EOS;
+        if (dens > 0.0) {
+          pr = gamma1 * uu * dens;
+          cs = sqrt(p.gamma * pr / dens);
+        }
+      }
+      a.f[F_UU][i] = uu;
+      a.f[F_PR][i] = pr;
+      a.f[F_CS][i] = cs;
+      if (p.imhd != 0) {
+        const double bx = a.f[F_BX][i], by = a.f[F_BY][i], bz = a.f[F_BZ][i];
+        const double B2 = bx * bx + by * by + bz * bz;
+        atomic_max_pos(&a.sc->stressmax, fmax(0.5 * B2 - pr, 0.0));
+      }
+      atomic_max_pos(&a.sc->hmax_nr, h);
+      atomicMax(&a.sc->itsmax, its);
+      atomicMin(&a.sc->nmin, nn);
+      atomicMax(&a.sc->nmax, nn);
+    }
+    atomicAdd(&a.sc->pairs_density, npairs);
+    atomicAdd(&a.sc->ncalc_total, (unsigned long long)(its - its0 + 1));
+    if (err) atomicCAS(&a.sc->err, 0, err);
+  }
+  // append unconverged particles to the next pass's list (one atomic per wave)
+  if (!persistent) {
+    const unsigned long long m = __ballot(unconverged);
+    const int lane = threadIdx.x & 63;
+    const int cnt = __popcll(m);
+    int base = 0;
+    if (lane == 0 && cnt) base = atomicAdd(&a.sc->ncalc_next, cnt);
+    base = __shfl(base, 0);
+    if (unconverged) next_list[base + __popcll(m & ((1ull << lane) - 1ull))] = k;
+  }
+}
+
+// P4.w = 1/h of every particle (before a symmetric pass after the first)
+__global__ void k_refresh_hinv(KArgs a, Phys p, int refresh_dxb) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.n) return;
+  const double h = a.f[F_H][a.sidx[k]];
+  a.P4[k].w = 1.0 / h;
+  if (refresh_dxb) a.dxb[k] = p.radkern * h;
+}
+
+// max h over all particles -> hmax_cur (and the ghost extent on a relink)
+__global__ void k_hmax(KArgs a, DScal* sc, int relink) {
+  __shared__ double sm[BS];
+  double hm = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x)
+    hm = fmax(hm, a.f[F_H][i]);
+  sm[threadIdx.x] = hm;
+  __syncthreads();
+  for (int s = BS / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomic_max_pos(&sc->hmax_cur, sm[0]);
+    if (relink) atomic_max_pos(&sc->hhmax_ghost, sm[0]);
+  }
+}
+
+__global__ void k_next_pass_scalars(DScal* sc, int relink) {
+  if (threadIdx.x | blockIdx.x) return;
+  sc->ncalc_next = 0;
+  sc->relink = 0;
+  sc->hmax_cur = 0.0;
+  if (relink) sc->hhmax_ghost = 0.0;
+}
+
+// ===========================================================================
+// force: fused pair sums of get_rates
+// ===========================================================================
+
+template <typename PT>
+__device__ __forceinline__ PT wave_max(PT v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+template <typename PT>
+__device__ __forceinline__ PT wave_min(PT v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+  return v;
+}
+
+template <int NDIM, typename PT>
+__global__ __launch_bounds__(BS) void k_force(KArgs a, Phys p, Grid g) {
+  __shared__ PT tg[NDSP_IKERN + 1];
+  {
+    const PT* src = sizeof(PT) == 4 ? (const PT*)a.tabf : (const PT*)a.tabd;
+    for (int t = threadIdx.x; t <= NDSP_IKERN; t += blockDim.x) tg[t] = src[NDSP_IKERN + 1 + t];
+  }
+  __syncthreads();
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = k < a.n;
+  typedef typename V4<PT>::type T4;
+  const T4* G0 = (const T4*)a.G0;
+  const T4* G1 = (const T4*)a.G1;
+  const T4* G2 = (const T4*)a.G2;
+  const T4* G3 = (const T4*)a.G3;
+  const T4* G4 = (const T4*)a.G4;
+  const PT* G5 = (const PT*)a.G5;
+  const bool mhd = p.imhd != 0;
+  const bool dust = p.idust != 0;
+  const PT ddq2 = (PT)p.ddq2table, dq2 = (PT)p.dq2table;
+  const double R2 = p.radkern2;
+  const PT beta = (PT)p.beta;
+
+  PT vsigmax_l = PT(0);
+  double dtc_l = 1.0e6;
+  unsigned long long npairs = 0;
+  int err = 0;
+  if (active) {
+    const int i = a.sidx[k];
+    const double4 pk = a.P4[k];
+    const double xi[3] = {pk.x, pk.y, pk.z};
+    const double hi1 = pk.w;
+    const double hi21 = mul_exact(hi1, hi1);
+    const T4 g0 = G0[k], g1 = G1[k], g2 = G2[k], g4 = G4[k];
+    const PT vi[3] = {g0.x, g0.y, g0.z};
+    const PT mi = g0.w;
+    const PT rhoi = g1.x, rho1i = PT(1) / rhoi, rho21i = rho1i * rho1i;
+    const PT pri = fmax(g1.y, PT(0));
+    const PT Prho2i = pri * rho21i;
+    const PT csi = g1.z, gradhi = g1.w;
+    const PT uui = g2.x, alphai = g2.y, alphaui = g2.z, alphaBi = g2.w;
+    const PT hi = g4.x;
+    const PT hi1p = (PT)hi1;
+    const PT hfacgrkerni = pown_t<NDIM>(hi1p) * hi1p;
+    PT Bi[3] = {0, 0, 0}, psii = 0;
+    if (mhd) { const T4 g3 = G3[k]; Bi[0] = g3.x; Bi[1] = g3.y; Bi[2] = g3.z; psii = g3.w; }
+    const PT Brhoi[3] = {Bi[0] * rho1i, Bi[1] * rho1i, Bi[2] * rho1i};
+    const PT B2i = Bi[0] * Bi[0] + Bi[1] * Bi[1] + Bi[2] * Bi[2];
+    const PT Brho2i = B2i * rho21i;
+    const PT valfven2i = B2i * rho1i;
+    const PT epsi = dust ? g4.z : PT(0);
+    const PT rhogasi = dust ? g4.w : rhoi;
+    const PT rhodusti = dust ? G5[k] : PT(0);
+    const PT stressmax = (PT)a.sc->stressmax;
+    const PT tstop = (PT)p.Kdrag;
+    const double rad = p.radkern * a.sc->hmax_nr;
+    if (!(rad < p.boxmin)) err = NDSP_ERR_CELLS;
+
+    Acc<PT> fx, fy, fz, dudt, dbx, dby, dbz, divb, cbx, cby, cbz, gpx, gpy, gpz, del2u, deps;
+    fx.zero(); fy.zero(); fz.zero(); dudt.zero(); dbx.zero(); dby.zero(); dbz.zero();
+    divb.zero(); cbx.zero(); cby.zero(); cbz.zero(); gpx.zero(); gpy.zero(); gpz.zero();
+    del2u.zero(); deps.zero();
+
+    if (!err) for_each_candidate<NDIM>(a, g, xi, rad, [&](int kk, const double* xj, const int* flip, bool img) {
+      double dx[3] = {0.0, 0.0, 0.0};
+      for (int d = 0; d < NDIM; d++) dx[d] = xi[d] - xj[d];
+      const double rij2 = rij2_exact<NDIM>(dx);
+      const double q2i = mul_exact(rij2, hi21);
+      const double hj1 = a.P4[kk].w;
+      const double q2j = mul_exact(mul_exact(rij2, hj1), hj1);
+      if (!(q2i < R2 || q2j < R2)) return;
+      if (kk == k && !img) return;
+      npairs++;
+      if (rij2 <= 0.0) { err = NDSP_ERR_COINCIDENT; return; }
+      const PT rij = sqrt((PT)rij2);
+      const PT rinv = PT(1) / rij;
+      PT dr[3] = {0, 0, 0};
+      for (int d = 0; d < NDIM; d++) dr[d] = (PT)dx[d] * rinv;
+      const T4 j0 = G0[kk], j1 = G1[kk], j2 = G2[kk], j4 = G4[kk];
+      const PT hj1p = (PT)hj1;
+      const PT hfacgrkernj = pown_t<NDIM>(hj1p) * hj1p;
+      const PT grkerni = ktab(tg, (PT)q2i, ddq2, dq2) * hfacgrkerni * gradhi;
+      const PT grkernj = ktab(tg, (PT)q2j, ddq2, dq2) * hfacgrkernj * j1.w;
+      const PT grkern = PT(0.5) * (grkerni + grkernj);
+      const PT vj[3] = {flip[0] ? -j0.x : j0.x, flip[1] ? -j0.y : j0.y, flip[2] ? -j0.z : j0.z};
+      const PT mj = j0.w;
+      const PT dvel[3] = {vi[0] - vj[0], vi[1] - vj[1], vi[2] - vj[2]};
+      const PT dvdotr = dvel[0] * dr[0] + dvel[1] * dr[1] + dvel[2] * dr[2];
+      const PT rhoj = j1.x, rho1j = PT(1) / rhoj, rho21j = rho1j * rho1j;
+      const PT rhoav1 = PT(0.5) * (rho1i + rho1j);
+      const PT prj = fmax(j1.y, PT(0));
+      const PT Prho2j = prj * rho21j;
+      const PT csj = j1.z;
+      const PT hj = j4.x;
+      PT Bj[3] = {0, 0, 0}, psij = 0, dB[3] = {0, 0, 0};
+      PT projBi = 0, projBj = 0, projdB = 0, projBrhoi = 0, projBrhoj = 0, Brho2j = 0, valfven2j = 0;
+      PT Brhoj[3] = {0, 0, 0};
+      if (mhd) {
+        const T4 j3 = G3[kk];
+        Bj[0] = j3.x; Bj[1] = j3.y; Bj[2] = j3.z; psij = j3.w;
+        for (int c = 0; c < 3; c++) { Brhoj[c] = Bj[c] * rho1j; dB[c] = Bi[c] - Bj[c]; }
+        projBi = Bi[0] * dr[0] + Bi[1] * dr[1] + Bi[2] * dr[2];
+        projBj = Bj[0] * dr[0] + Bj[1] * dr[1] + Bj[2] * dr[2];
+        projdB = dB[0] * dr[0] + dB[1] * dr[1] + dB[2] * dr[2];
+        projBrhoi = Brhoi[0] * dr[0] + Brhoi[1] * dr[1] + Brhoi[2] * dr[2];
+        projBrhoj = Brhoj[0] * dr[0] + Brhoj[1] * dr[1] + Brhoj[2] * dr[2];
+        const PT B2j = Bj[0] * Bj[0] + Bj[1] * Bj[1] + Bj[2] * Bj[2];
+        valfven2j = B2j / j4.y;
+        Brho2j = B2j * rho21j;
+      }
+      // signal velocities, src/ratesND_mhd.f90:1417-1482
+      PT vsigi, vsigj, vsigB;
+      if (mhd) {
+        const PT vsig2i = csi * csi + valfven2i;
+        const PT vsig2j = csj * csj + valfven2j;
+        const PT ti = csi * projBi, tj = csj * projBj;
+        const PT vsigproji = vsig2i * vsig2i - PT(4) * (ti * ti) * rho1i;
+        const PT vsigprojj = vsig2j * vsig2j - PT(4) * (tj * tj) * rho1j;
+        vsigi = sqrt(PT(0.5) * (vsig2i + sqrt(fmax(vsigproji, PT(0)))));
+        vsigj = sqrt(PT(0.5) * (vsig2j + sqrt(fmax(vsigprojj, PT(0)))));
+        if (vsigproji < PT(-1.e-4) * vsig2i * vsig2i || vsigprojj < PT(-1.e-4) * vsig2j * vsig2j)
+          err = NDSP_ERR_VSIG;
+        if (p.iavlim[2] != 2) vsigB = sqrt(dvel[0] * dvel[0] + dvel[1] * dvel[1] + dvel[2] * dvel[2]);
+        else vsigB = PT(0.5) * (vsigi + vsigj) + fabs(dvdotr);
+      } else {
+        vsigi = csi;
+        vsigj = csj;
+        vsigB = PT(0);
+      }
+      const PT vsig = PT(0.5) * fmax(vsigi + vsigj - beta * dvdotr, PT(0));
+      const PT vsigu = sqrt(fabs(pri - prj) * rhoav1);
+      const PT vsigdtc = fmax(fmax(vsig, PT(0.5) * (vsigi + vsigj + beta * fabs(dvdotr))), vsigB);
+      vsigmax_l = fmax(vsigmax_l, vsigdtc);
+      if (vsigdtc > PT(1.e-10)) dtc_l = fmin(dtc_l, (double)(fmin(hi, hj) / vsigdtc));
+      PT f[3] = {0, 0, 0};
+      PT du = PT(0);
+      // artificial dissipation
+      if (dust) {
+        // src/ratesND_mhd.f90:2154-2206
+        if (dvdotr < PT(0)) {
+          const PT vsi = alphai * csi - beta * dvdotr;
+          const PT vsj = j2.y * csj - beta * dvdotr;
+          const PT qi = -rhogasi * vsi * dvdotr;
+          const PT qj = -(dust ? j4.w : rhoj) * vsj * dvdotr;
+          const PT visc = PT(0.5) * (qi * rho21i * grkerni + qj * rho21j * grkernj);
+          for (int c = 0; c < 3; c++) f[c] -= mj * visc * dr[c];
+          const PT duu = uui - j2.x;
+          const PT cfaci = PT(0.5) * alphaui * rhoi * vsigu * duu;
+          const PT cfacj = PT(0.5) * j2.z * rhoj * vsigu * duu;
+          const PT diffu = cfaci * grkerni * rho21i + cfacj * grkernj * rho21j;
+          du += PT(0.5) * qi * rho1i / rhogasi * mj * dvdotr * grkerni + mj * diffu / (PT(1) - epsi);
+        }
+      } else {
+        // src/ratesND_mhd.f90:1700-1894
+        const PT alphaav = PT(0.5) * (alphai + j2.y);
+        const PT alphau = PT(0.5) * (alphaui + j2.z);
+        const PT alphaB = PT(0.5) * (alphaBi + j2.w);
+        const PT term = vsig * rhoav1 * grkern;
+        const PT termu = vsigu * rhoav1 * grkern;
+        const PT termB = vsigB * rhoav1 * grkern;
+        if (dvdotr < PT(0)) {
+          const PT visc = alphaav * term * (-dvdotr);
+          for (int c = 0; c < 3; c++) f[c] -= mj * visc * dr[c];
+        }
+        if (mhd) {
+          const PT fac = rhoi * mj * alphaB * termB * rhoav1;
+          dbx.add(fac * dB[0]);
+          dby.add(fac * dB[1]);
+          dbz.add(fac * dB[2]);
+        }
+        if (p.iener > 0) {
+          PT vissv = PT(0);
+          if (dvdotr < PT(0)) {
+            const PT t = (vi[0] * dr[0] + vi[1] * dr[1] + vi[2] * dr[2]) - (vj[0] * dr[0] + vj[1] * dr[1] + vj[2] * dr[2]);
+            vissv = -alphaav * PT(0.5) * (t * t);
+          }
+          const PT vissu = alphau * (uui - j2.x);
+          const PT vissB = mhd ? -alphaB * PT(0.5) * (dB[0] * dB[0] + dB[1] * dB[1] + dB[2] * dB[2]) * rhoav1 : PT(0);
+          du += mj * (term * vissv + termu * vissu + termB * vissB);
+        }
+      }
+      // pressure, src/ratesND_mhd.f90:1538-1568
+      {
+        const PT prterm = Prho2i * grkerni + Prho2j * grkernj;
+        for (int c = 0; c < 3; c++) f[c] -= mj * prterm * dr[c];
+      }
+      if (mhd) {
+        // mhd_terms, src/ratesND_mhd.f90:2377-2720 (tensor force, imhd 11)
+        const PT fiso = PT(0.5) * (Brho2i * grkerni + Brho2j * grkernj);
+        for (int c = 0; c < 3; c++) {
+          const PT faniso = (Brhoi[c] * projBrhoi - stressmax * dr[c] * rho21i) * grkerni +
+                            (Brhoj[c] * projBrhoj - stressmax * dr[c] * rho21j) * grkernj;
+          f[c] += mj * (faniso - fiso * dr[c]);
+        }
+        divb.add(-mj * projdB * grkern);
+        const PT cb[3] = {dB[1] * dr[2] - dB[2] * dr[1], dB[2] * dr[0] - dB[0] * dr[2], dB[0] * dr[1] - dB[1] * dr[0]};
+        cbx.add(mj * cb[0] * grkern);
+        cby.add(mj * cb[1] * grkern);
+        cbz.add(mj * cb[2] * grkern);
+        const PT ind = mj * projBrhoi * grkerni;
+        dbx.add(-ind * dvel[0]);
+        dby.add(-ind * dvel[1]);
+        dbz.add(-ind * dvel[2]);
+        if (p.idivbzero >= 2) {
+          const PT gpt = mj * (psii * rho21i * grkerni + psij * rho21j * grkernj);
+          gpx.add(-gpt * dr[0]);
+          gpy.add(-gpt * dr[1]);
+          gpz.add(-gpt * dr[2]);
+        }
+      }
+      if (dust) {
+        // dust_derivs_diffusion, idustevol 0, src/ratesND_mhd.f90:2813-2878
+        const PT Di = rhodusti * rho1i * tstop;
+        const PT Dj = G5[kk] * rho1j * tstop;
+        const PT Dav = PT(0.5) * (Di + Dj);
+        const PT grgrkern = PT(-2) * grkern * rinv;
+        const PT diffterm = rho1i * rho1j * Dav * (pri - prj) * grgrkern;
+        deps.add(mj * diffterm);
+        if (p.iener > 0) {
+          const PT duu = uui - j2.x;
+          du += pri * rho1i / rhogasi * mj * dvdotr * grkerni - PT(0.5) * mj * duu * diffterm / (PT(1) - epsi);
+        }
+      }
+      fx.add(f[0]);
+      fy.add(f[1]);
+      fz.add(f[2]);
+      dudt.add(du);
+      if (p.iavlim[1] > 0) del2u.add(mj * rho1j * ((uui - j2.x) * rinv) * grkerni);
+    });
+    a.f[F_FX][i] = fx.value();
+    a.f[F_FY][i] = fy.value();
+    a.f[F_FZ][i] = fz.value();
+    a.f[F_DUDT][i] = dudt.value();
+    a.f[F_DBX][i] = dbx.value();
+    a.f[F_DBY][i] = dby.value();
+    a.f[F_DBZ][i] = dbz.value();
+    a.f[F_DIVB][i] = divb.value();
+    a.f[F_CBX][i] = cbx.value();
+    a.f[F_CBY][i] = cby.value();
+    a.f[F_CBZ][i] = cbz.value();
+    a.f[F_GPX][i] = gpx.value();
+    a.f[F_GPY][i] = gpy.value();
+    a.f[F_GPZ][i] = gpz.value();
+    a.f[F_DEL2U][i] = del2u.value();
+    a.f[F_DEPS][i] = deps.value();
+  }
+  // one atomic per wavefront for the global extrema and counters
+  PT vm = wave_max(vsigmax_l);
+  double dm = wave_min(dtc_l);
+  unsigned long long np = npairs;
+  for (int o = 32; o > 0; o >>= 1) np += __shfl_xor(np, o);
+  if ((threadIdx.x & 63) == 0) {
+    atomic_max_pos(&a.sc->vsigmax, (double)vm);
+    atomic_min_pos(&a.sc->dtcourant, dm);
+    atomicAdd(&a.sc->pairs_force, np);
+  }
+  if (err) atomicCAS(&a.sc->err, 0, err);
+}
+
+// per-particle finalisation, src/ratesND_mhd.f90:482-943 (FP64)
+__global__ void k_finalize(KArgs a, Phys p) {
+#pragma clang fp contract(off)
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double fh = 0.0, dtd = DBL_MAX;
+  if (i < a.n) {
+    double** f = a.f;
+    const double rhoi = f[F_RHO][i];
+    const double rho1i = 1.0 / rhoi;
+    const double h = f[F_H][i];
+    const double cs = f[F_CS][i];
+    if (p.idust == 4) {
+      const double rhodusti = f[F_RHODUST][i], rhogasi = f[F_RHOGAS][i];
+      const double dustfraci = rhodusti * rho1i;
+      const double ts = p.Kdrag;
+      f[F_DVX][i] = rhogasi > DBL_MIN ? -rhoi / rhogasi * f[F_FX][i] * ts : 0.0;
+      f[F_DVY][i] = rhogasi > DBL_MIN ? -rhoi / rhogasi * f[F_FY][i] * ts : 0.0;
+      f[F_DVZ][i] = rhogasi > DBL_MIN ? -rhoi / rhogasi * f[F_FZ][i] * ts : 0.0;
+      dtd = fabs(0.5 * (h * h) / (dustfraci * ts * (cs * cs)));
+    }
+    const bool mhd = p.imhd != 0;
+    if (mhd) {
+      f[F_CBX][i] *= rho1i;
+      f[F_CBY][i] *= rho1i;
+      f[F_CBZ][i] *= rho1i;
+      f[F_DIVB][i] *= rho1i;
+    }
+    const double fxv = f[F_FX][i], fyv = f[F_FY][i], fzv = f[F_FZ][i];
+    fh = sqrt(fxv * fxv + fyv * fyv + fzv * fzv) / h;
+    double valfven2i = 0.0;
+    const double bx = f[F_BX][i], by = f[F_BY][i], bz = f[F_BZ][i];
+    if (mhd) valfven2i = (bx * bx + by * by + bz * bz) / f[F_DENS][i];
+    const double vsig = sqrt(cs * cs + valfven2i);
+    if (p.imhd == 11) {
+      const double drhodt = f[F_DRHODT][i];
+      double dbx = f[F_DBX][i] + bx * rho1i * drhodt;
+      double dby = f[F_DBY][i] + by * rho1i * drhodt;
+      double dbz = f[F_DBZ][i] + bz * rho1i * drhodt;
+      if (p.idivbzero >= 2) {
+        const double gx = f[F_GPX][i] * rhoi, gy = f[F_GPY][i] * rhoi, gz = f[F_GPZ][i] * rhoi;
+        f[F_GPX][i] = gx; f[F_GPY][i] = gy; f[F_GPZ][i] = gz;
+        dbx = dbx + gx; dby = dby + gy; dbz = dbz + gz;
+      }
+      f[F_DBX][i] = dbx; f[F_DBY][i] = dby; f[F_DBZ][i] = dbz;
+    }
+    if (p.iener > 0 && p.idust == 0) {
+      f[F_DUDT][i] = f[F_DUDT][i] + f[F_PR][i] * (rho1i * rho1i) * f[F_DRHODT][i];
+    }
+    f[F_DENDT][i] = f[F_DUDT][i];
+    double da0 = 0.0, da1 = 0.0, da2 = 0.0;
+    if (p.iavlim[0] || p.iavlim[1] || p.iavlim[2]) {
+      const double tdecay1 = (p.avdecayconst * vsig) / h;
+      const double al0 = f[F_A0][i], al1 = f[F_A1][i], al2 = f[F_A2][i];
+      if (p.iavlim[0] == 1 || p.iavlim[0] == 2) {
+        double source = fmax(f[F_DRHODT][i] * rho1i, 0.0);
+        if (p.iavlim[0] == 2) source = source * (2.0 - al0);
+        da0 = (p.alphamin - al0) * tdecay1 + p.avfact * source;
+      }
+      if (p.iener > 0 && p.iavlim[1] > 0) {
+        const double uu = f[F_UU][i];
+        const double sourceu = uu > DBL_EPSILON ? h * fabs(f[F_DEL2U][i]) / sqrt(uu) : 0.0;
+        da1 = (p.alphaumin - al1) * tdecay1 + sourceu;
+      }
+      if (p.iavlim[2] != 0 && mhd) {
+        const double c0 = f[F_CBX][i], c1 = f[F_CBY][i], c2 = f[F_CBZ][i];
+        const double sourceJ = sqrt((c0 * c0 + c1 * c1 + c2 * c2) * rho1i);
+        const double sourcedivB = 10. * fabs(f[F_DIVB][i]) * sqrt(rho1i);
+        double sourceB = fmax(sourceJ, sourcedivB);
+        if (p.iavlim[2] == 3) sourceB = sqrt(fmax(f[F_DRHODT][i] * rho1i, 0.0) * (2. - al2) * sourceB);
+        da2 = (p.alphabmin - al2) * tdecay1 + sourceB;
+      }
+    }
+    f[F_DA0][i] = da0;
+    f[F_DA1][i] = da1;
+    f[F_DA2][i] = da2;
+    if (p.idivbzero >= 2 && p.idivbzero <= 7) {
+      const double vsigmax = a.sc->vsigmax;
+      f[F_DPSIDT][i] = -(vsigmax * vsigmax) * f[F_DIVB][i] - p.psidecayfact * f[F_PSI][i] * vsigmax / h;
+    } else {
+      f[F_DPSIDT][i] = 0.0;
+    }
+  }
+  double fm = wave_max(fh);
+  double dm = wave_min(dtd);
+  if ((threadIdx.x & 63) == 0) {
+    atomic_max_pos(&a.sc->fhmax, fm);
+    atomic_min_pos(&a.sc->dtdrag, dm);
+  }
+}
+
+// timestep, src/ratesND_mhd.f90:938-943 and src/stepND_leapfrog_mhd.f90:252
+__global__ void k_dt(DScal* sc, Phys p) {
+  if (threadIdx.x | blockIdx.x) return;
+  double dtforce = DBL_MAX;
+  if (sc->fhmax > 0.0) dtforce = fmin(dtforce, sqrt(1.0 / sc->fhmax));
+  sc->dtforce = dtforce;
+  sc->dt = fmin(fmin(fmin(p.C_force * dtforce, p.C_cour * sc->dtcourant), 0.9 * sc->dtdrag),
+                p.C_force * DBL_MAX);
+}
+
+__global__ void k_init_density_scalars(DScal* sc, double bconstmax) {
+  if (threadIdx.x | blockIdx.x) return;
+  sc->hmax_nr = 0.0;
+  sc->hmax_cur = sc->hmax_bin;
+  sc->hhmax_ghost = sc->hmax_bin;
+  sc->ncalc_next = 0;
+  sc->relink = 0;
+  sc->stressmax = fmax(0.0, bconstmax);
+  sc->itsmax = 0;
+  sc->nmin = INT_MAX;
+  sc->nmax = 0;
+  sc->pairs_density = 0;
+  sc->ncalc_total = 0;
+}
+
+__global__ void k_init_force_scalars(DScal* sc) {
+  if (threadIdx.x | blockIdx.x) return;
+  sc->vsigmax = 0.0;
+  sc->dtcourant = 1.0e6;
+  sc->fhmax = 0.0;
+  sc->dtdrag = DBL_MAX;
+  sc->pairs_force = 0;
+}
+
+// ===========================================================================
+// leapfrog, src/stepND_leapfrog_mhd.f90:31-275
+// ===========================================================================
+
+__global__ void k_predict(KArgs a, Phys p, double dt) {
+#pragma clang fp contract(off)
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  double** f = a.f;
+  static const int xs[3] = {F_X, F_Y, F_Z}, vs[3] = {F_VX, F_VY, F_VZ}, bs[3] = {F_BX, F_BY, F_BZ};
+  static const int as[3] = {F_A0, F_A1, F_A2}, fs[3] = {F_FX, F_FY, F_FZ}, dbs[3] = {F_DBX, F_DBY, F_DBZ};
+  static const int das[3] = {F_DA0, F_DA1, F_DA2};
+  for (int c = 0; c < 3; c++) {
+    f[F_XIN + c][i] = f[xs[c]][i];
+    f[F_VXIN + c][i] = f[vs[c]][i];
+    f[F_BXIN + c][i] = f[bs[c]][i];
+    f[F_A0IN + c][i] = f[as[c]][i];
+    f[F_FXIN + c][i] = f[fs[c]][i];
+    f[F_DBXIN + c][i] = f[dbs[c]][i];
+    f[F_DA0IN + c][i] = f[das[c]][i];
+  }
+  f[F_HIN][i] = f[F_H][i];
+  f[F_ENIN][i] = f[F_EN][i];
+  f[F_PSIIN][i] = f[F_PSI][i];
+  f[F_EPSIN][i] = f[F_EPS][i];
+  f[F_DHDTIN][i] = f[F_DHDT][i];
+  f[F_DENDTIN][i] = f[F_DENDT][i];
+  f[F_DPSIDTIN][i] = f[F_DPSIDT][i];
+  f[F_DEPSIN][i] = f[F_DEPS][i];
+  for (int d = 0; d < p.ndim; d++)
+    f[xs[d]][i] = f[F_XIN + d][i] + dt * f[F_VXIN + d][i] + 0.5 * dt * dt * f[F_FXIN + d][i];
+  for (int c = 0; c < 3; c++) f[vs[c]][i] = (f[F_VXIN + c][i] + dt * f[F_FXIN + c][i]) / 1.0;
+  if (p.imhd != 0)
+    for (int c = 0; c < 3; c++) f[bs[c]][i] = f[F_BXIN + c][i] + dt * f[F_DBXIN + c][i];
+  f[F_H][i] = f[F_HIN][i] + dt * f[F_DHDTIN][i];
+  if (p.iener != 0) f[F_EN][i] = f[F_ENIN][i] + dt * f[F_DENDTIN][i];
+  for (int c = 0; c < 3; c++)
+    if (p.iavlim[c] != 0) f[as[c]][i] = fmin(f[F_A0IN + c][i] + dt * f[F_DA0IN + c][i], 1.0);
+  if (p.idivbzero >= 2) f[F_PSI][i] = f[F_PSIIN][i] + dt * f[F_DPSIDTIN][i];
+  if (p.idust) f[F_EPS][i] = f[F_EPSIN][i] + dt * f[F_DEPSIN][i];
+}
+
+__global__ void k_correct(KArgs a, Phys p, double dt) {
+#pragma clang fp contract(off)
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  double** f = a.f;
+  const double hdt = 0.5 * dt;
+  static const int vs[3] = {F_VX, F_VY, F_VZ}, bs[3] = {F_BX, F_BY, F_BZ}, as[3] = {F_A0, F_A1, F_A2};
+  static const int fs[3] = {F_FX, F_FY, F_FZ}, dbs[3] = {F_DBX, F_DBY, F_DBZ}, das[3] = {F_DA0, F_DA1, F_DA2};
+  for (int c = 0; c < 3; c++)
+    f[vs[c]][i] = (f[F_VXIN + c][i] + hdt * (f[fs[c]][i] + f[F_FXIN + c][i])) / 1.0;
+  if (p.imhd != 0)
+    for (int c = 0; c < 3; c++) f[bs[c]][i] = f[F_BXIN + c][i] + hdt * (f[dbs[c]][i] + f[F_DBXIN + c][i]);
+  const double hnew = f[F_HIN][i] + hdt * (f[F_DHDT][i] + f[F_DHDTIN][i]);
+  f[F_H][i] = hnew;
+  if (hnew <= 0.0) atomicCAS(&a.sc->err, 0, NDSP_ERR_H_NONPOSITIVE);
+  if (p.iener != 0) f[F_EN][i] = f[F_ENIN][i] + hdt * (f[F_DENDT][i] + f[F_DENDTIN][i]);
+  for (int c = 0; c < 3; c++)
+    if (p.iavlim[c] != 0) f[as[c]][i] = fmin(f[F_A0IN + c][i] + hdt * (f[das[c]][i] + f[F_DA0IN + c][i]), 1.0);
+  if (p.idivbzero >= 2) f[F_PSI][i] = f[F_PSIIN][i] + hdt * (f[F_DPSIDT][i] + f[F_DPSIDTIN][i]);
+  if (p.idust) f[F_EPS][i] = f[F_EPSIN][i] + hdt * (f[F_DEPS][i] + f[F_DEPSIN][i]);
+}
+
+// ===========================================================================
+// FP64 conserved-quantity reduction, src/evwrite_mhd.f90:124-248
+// ===========================================================================
+
+#define NCONS 17
+__global__ void k_conserved(KArgs a, Phys p, double* part) {
+  __shared__ double sm[NCONS][BS];
+  int t = threadIdx.x;
+  double v[NCONS];
+  for (int q = 0; q < NCONS; q++) v[q] = 0.0;
+  v[12] = DBL_MAX;
+  v[13] = -DBL_MAX;
+  double** f = a.f;
+  for (int i = blockIdx.x * blockDim.x + t; i < a.n; i += gridDim.x * blockDim.x) {
+    const double m = f[F_M][i];
+    const double vx = f[F_VX][i], vy = f[F_VY][i], vz = f[F_VZ][i];
+    const double x = f[F_X][i], y = p.ndim >= 2 ? f[F_Y][i] : 0.0, z = p.ndim >= 3 ? f[F_Z][i] : 0.0;
+    v[4] += m * vx; v[5] += m * vy; v[6] += m * vz;
+    if (p.ndim == 3) {
+      v[8] += m * (y * vz - z * vy);
+      v[9] += m * (z * vx - x * vz);
+      v[10] += m * (x * vy - y * vx);
+    } else if (p.ndim == 2) {
+      v[10] += m * (x * vy - y * vx);
+    }
+    v[0] += 0.5 * m * (vx * vx + vy * vy + vz * vz);
+    const double rho = f[F_RHO][i];
+    if (p.idust) {
+      const double eps = f[F_DUSTFRAC][i], dterm = 1.0 - eps;
+      const double dx = f[F_DVX][i], dy = f[F_DVY][i], dz = f[F_DVZ][i];
+      v[0] += 0.5 * m * eps * dterm * (dx * dx + dy * dy + dz * dz);
+      v[1] += m * f[F_UU][i] * dterm;
+      v[15] += m * dterm;
+      v[16] += m * eps;
+    } else {
+      v[1] += m * f[F_UU][i];
+    }
+    if (p.imhd) {
+      const double bx = f[F_BX][i], by = f[F_BY][i], bz = f[F_BZ][i];
+      v[2] += 0.5 * m * (bx * bx + by * by + bz * bz) / rho;
+    }
+    v[12] = fmin(v[12], rho);
+    v[13] = fmax(v[13], rho);
+    v[14] += rho;
+  }
+  for (int q = 0; q < NCONS; q++) sm[q][t] = v[q];
+  __syncthreads();
+  for (int s = BS / 2; s > 0; s >>= 1) {
+    if (t < s) {
+      for (int q = 0; q < NCONS; q++) {
+        if (q == 12) sm[q][t] = fmin(sm[q][t], sm[q][t + s]);
+        else if (q == 13) sm[q][t] = fmax(sm[q][t], sm[q][t + s]);
+        else sm[q][t] += sm[q][t + s];
+      }
+    }
+    __syncthreads();
+  }
+  if (t < NCONS) part[blockIdx.x * NCONS + t] = sm[t][0];
+}
+
+__global__ void k_conserved_final(const double* part, int nb, double* out, int n) {
+  if (threadIdx.x | blockIdx.x) return;
+  double v[NCONS];
+  for (int q = 0; q < NCONS; q++) v[q] = 0.0;
+  v[12] = DBL_MAX;
+  v[13] = -DBL_MAX;
+  for (int b = 0; b < nb; b++)
+    for (int q = 0; q < NCONS; q++) {
+      const double x = part[b * NCONS + q];
+      if (q == 12) v[q] = fmin(v[q], x);
+      else if (q == 13) v[q] = fmax(v[q], x);
+      else v[q] += x;
+    }
+  v[3] = v[0] + v[2] + v[1];
+  v[7] = sqrt(v[4] * v[4] + v[5] * v[5] + v[6] * v[6]);
+  v[11] = sqrt(v[8] * v[8] + v[9] * v[9] + v[10] * v[10]);
+  v[14] = v[14] / (double)n;
+  for (int q = 0; q < NCONS; q++) out[q] = v[q];
+}
+
+// ===========================================================================
+// reorder of the particle state into cell order
+// ===========================================================================
+
+__global__ void k_gather_d(const double* __restrict__ in, double* __restrict__ out, const int* sidx, int n) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = in[sidx[k]];
+}
+__global__ void k_gather_i(const int* __restrict__ in, int* __restrict__ out, const int* sidx, int n) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = in[sidx[k]];
+}
+
+// ===========================================================================
+// neighbour-set enumeration (test support)
+// ===========================================================================
+
+template <int NDIM>
+__global__ void k_nb(KArgs a, Phys p, Grid g, int sym, const int* offs, int* out) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.n) return;
+  const double4 pk = a.P4[k];
+  const double xi[3] = {pk.x, pk.y, pk.z};
+  const double hi21 = mul_exact(pk.w, pk.w);
+  const double R2 = p.radkern2;
+  const int i = a.sidx[k];
+  const double rad = p.radkern * (sym ? a.sc->hmax_nr : a.f[F_H][i]);
+  int cnt = 0;
+  if (!(rad < p.boxmin)) { if (!offs) out[k] = 0; return; }
+  int base = offs ? offs[k] : 0;
+  for_each_candidate<NDIM>(a, g, xi, rad, [&](int kk, const double* xj, const int* flip, bool img) {
+    double dx[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < NDIM; d++) dx[d] = xi[d] - xj[d];
+    const double rij2 = rij2_exact<NDIM>(dx);
+    const double q2i = mul_exact(rij2, hi21);
+    const double hj1 = a.P4[kk].w;
+    const double q2j = mul_exact(mul_exact(rij2, hj1), hj1);
+    const bool in = sym ? (q2i < R2 || q2j < R2) : (q2i < R2);
+    if (!in) return;
+    if (offs) {
+      int code = 0;
+      if (img) {
+        const double4 pj = a.P4[kk];
+        const double xo[3] = {pj.x, pj.y, pj.z};
+        for (int d = NDIM - 1; d >= 0; d--) {
+          code *= 3;
+          if (xj[d] > xo[d]) code += 1;
+          if (xj[d] < xo[d]) code += 2;
+        }
+      }
+      out[3 * (base + cnt) + 0] = a.uid[i];
+      out[3 * (base + cnt) + 1] = a.uid[a.sidx[kk]];
+      out[3 * (base + cnt) + 2] = code;
+    }
+    cnt++;
+  });
+  if (!offs) out[k] = cnt;
+}
+
+// ===========================================================================
+// host side
+// ===========================================================================
+
+// kernel tables, src/kernelND.f90:127-161 (set-up), 211-269 (M6 quintic),
+// 4143-4190 (M4 cubic), 4279-4289 (normalisation)
+static int make_tables(int ikernel, int ndim, std::vector<double>& tab, double& radkern, double& dq2) {
+  const int n = NDSP_IKERN + 1;
+  tab.assign(3 * n, 0.0);
+  double* w = tab.data();
+  double* g = w + n;
+  double* gg = g + n;
+  const double pi = 3.141592653589;
+  double cnormk;
+  if (ikernel == 3) {
+    radkern = 3.0;
+    dq2 = radkern * radkern / (double)NDSP_IKERN;
+    cnormk = ndim == 1 ? 1. / 120. : ndim == 2 ? 7. / (478 * pi) : 1. / (120. * pi);
+    for (int i = 0; i < n; i++) {
+      double q2 = i * dq2, q4 = q2 * q2, q = std::sqrt(q2);
+      double term1 = -5. * std::pow(3. - q, 4.);
+      if (q < 1.0) {
+        w[i] = 66. - 60. * q2 + 30. * q4 - 10. * q4 * q;
+        g[i] = term1 + 30 * std::pow(2. - q, 4) - 75. * std::pow(1. - q, 4);
+        gg[i] = 20. * std::pow(3. - q, 3) - 120. * std::pow(2. - q, 3) + 300. * std::pow(1. - q, 3);
+      } else if (q < 2.0) {
+        w[i] = std::pow(3. - q, 5) - 6. * std::pow(2. - q, 5);
+        g[i] = term1 + 30 * std::pow(2. - q, 4);
+        gg[i] = 20. * std::pow(3. - q, 3) - 120. * std::pow(2. - q, 3);
+      } else if (q <= 3.0) {
+        w[i] = std::pow(3. - q, 5);
+        g[i] = term1;
+        gg[i] = 20. * std::pow(3. - q, 3);
+      }
+    }
+  } else if (ikernel == 0) {
+    radkern = 2.0;
+    dq2 = radkern * radkern / (double)NDSP_IKERN;
+    cnormk = ndim == 1 ? 0.66666666666 : ndim == 2 ? 10. / (7. * pi) : 1. / pi;
+    for (int i = 0; i < n; i++) {
+      double q2 = i * dq2, q = std::sqrt(q2);
+      if (q < 1.0) {
+        w[i] = 1. - 1.5 * q2 + 0.75 * q * q2;
+        g[i] = -3. * q + 2.25 * q2;
+        gg[i] = -3. + 4.5 * q;
+      } else if (q <= 2.0) {
+        double t = 2. - q;
+        w[i] = 0.25 * (t * t * t);
+        g[i] = -0.75 * (t * t);
+        gg[i] = 1.5 * t;
+      }
+    }
+  } else {
+    return -1;
+  }
+  for (int i = 0; i < 3 * n; i++) tab[i] = cnormk * tab[i];
+  return 0;
+}
+
+struct ndsp_ctx {
+  Phys p;
+  int prec;
+  int n;
+  int ibound[3];
+  double xmin[3], xmax[3];
+  double bconstmax;
+  hipStream_t st = nullptr;
+  KArgs a;
+  double* scratch_d = nullptr;  // one field of n doubles for reorders
+  int* scratch_i = nullptr;
+  int* bsum = nullptr;
+  int* list_a = nullptr;        // active lists of the h-rho passes
+  int* list_b = nullptr;
+  double* part = nullptr;       // reduction partials
+  double* cons_out = nullptr;
+  int nb_part = 0;
+  long long ncell_cap = 0;
+  Grid g;
+  bool grid_valid = false;
+  bool have_rates = false;
+  int nsteps = 0;
+  int reorder_every = 10;
+  double dt = 0.0, time = 0.0;
+  // timing
+  bool timing = false;
+  std::vector<std::string> tnames;
+  std::vector<hipEvent_t> ev0, ev1;
+  std::vector<double> tms;
+  std::vector<int> tcount;
+  std::vector<std::pair<int, int>> pending;  // (label, event pair index)
+  int nev = 0;
+};
+
+// ---- timing helpers ------------------------------------------------------
+static int label_id(ndsp_ctx* c, const char* name) {
+  for (size_t k = 0; k < c->tnames.size(); k++)
+    if (c->tnames[k] == name) return (int)k;
+  c->tnames.push_back(name);
+  c->tms.push_back(0.0);
+  c->tcount.push_back(0);
+  return (int)c->tnames.size() - 1;
+}
+
+struct TScope {
+  ndsp_ctx* c;
+  int lab, ev;
+  TScope(ndsp_ctx* c_, const char* name) : c(c_), lab(-1), ev(-1) {
+    if (!c->timing) return;
+    lab = label_id(c, name);
+    if (c->nev >= (int)c->ev0.size()) {
+      hipEvent_t a0, a1;
+      hipEventCreate(&a0);
+      hipEventCreate(&a1);
+      c->ev0.push_back(a0);
+      c->ev1.push_back(a1);
+    }
+    ev = c->nev++;
+    hipEventRecord(c->ev0[ev], c->st);
+  }
+  ~TScope() {
+    if (ev < 0) return;
+    hipEventRecord(c->ev1[ev], c->st);
+    c->pending.push_back({lab, ev});
+  }
+};
+
+static void timing_collect(ndsp_ctx* c) {
+  if (!c->timing) return;
+  for (auto& pe : c->pending) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->ev0[pe.second], c->ev1[pe.second]);
+    c->tms[pe.first] += ms;
+    c->tcount[pe.first] += 1;
+  }
+  c->pending.clear();
+  c->nev = 0;
+}
+
+static inline unsigned nblk(long long n, int bs = BS) { return (unsigned)((n + bs - 1) / bs); }
+
+template <typename F>
+static int dispatch_ndim(int ndim, F&& f) {
+  if (ndim == 1) return f(std::integral_constant<int, 1>());
+  if (ndim == 2) return f(std::integral_constant<int, 2>());
+  return f(std::integral_constant<int, 3>());
+}
+
+static int sync_scalars(ndsp_ctx* c, DScal& h) {
+  HIPCHK(hipMemcpyAsync(&h, c->a.sc, sizeof(DScal), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+static int check_err(ndsp_ctx* c, const DScal& h) {
+  if (h.err) {
+    static const char* names[] = {"ok", "h <= 0", "rho <= 0", "density not converged", "u < 0",
+                                  "cells", "outside", "alloc", "coincident particles",
+                                  "vsig determinant < 0"};
+    int e = h.err;
+    return set_err(e, std::string("device reported: ") + (e < 10 ? names[e] : "error"));
+  }
+  return 0;
+}
+
+// set_linklist: boundary, statistics, grid, counting sort, density view
+static int do_linklist(ndsp_ctx* c, int in_step) {
+  KArgs& a = c->a;
+  const int n = c->n;
+  HIPCHK(hipMemsetAsync(&a.sc->err, 0, sizeof(int), c->st));
+  {
+    TScope t(c, "boundary");
+    hipLaunchKernelGGL(k_boundary, dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p, c->g, in_step);
+  }
+  {
+    TScope t(c, "binstats");
+    hipLaunchKernelGGL(k_binstats, dim3(c->nb_part), dim3(BS), 0, c->st, a, c->p.ndim, c->part);
+    hipLaunchKernelGGL(k_binstats_final, dim3(1), dim3(1), 0, c->st, a.sc, c->part, c->nb_part);
+  }
+  DScal hs;
+  if (int e = sync_scalars(c, hs)) return e;
+  const double hmax = hs.hmax_bin;
+  const double cs = c->p.radkern * hmax;
+  if (!(cs > 0.0) || !std::isfinite(cs)) return set_err(NDSP_ERR_CELLS, "link: max h <= 0");
+  Grid g = c->g;
+  long long ncell = 1;
+  for (int d = 0; d < 3; d++) {
+    if (d >= c->p.ndim) {
+      g.nc[d] = 1; g.lo[d] = 0.0; g.w[d] = 1.0; g.inv_w[d] = 1.0;
+      continue;
+    }
+    if (g.bound[d] >= 2) {
+      const double L = g.xmax[d] - g.xmin[d];
+      if (L < cs) return set_err(NDSP_ERR_CELLS, "link: radkern*hmax exceeds the box");
+      double ncd = std::floor(L / cs);
+      if (ncd > 1.0e8) return set_err(NDSP_ERR_CELLS, "link: too many cells");
+      g.nc[d] = std::max(1, (int)ncd);
+      g.lo[d] = g.xmin[d];
+      g.w[d] = L / g.nc[d];
+    } else {
+      const double ext = hs.bbox_hi[d] - hs.bbox_lo[d];
+      double ncd = std::floor(ext / cs) + 1.0;
+      if (ncd > 1.0e8) return set_err(NDSP_ERR_CELLS, "link: too many cells");
+      g.nc[d] = (int)ncd;
+      g.lo[d] = hs.bbox_lo[d];
+      g.w[d] = cs;
+    }
+    g.inv_w[d] = 1.0 / g.w[d];
+    ncell *= g.nc[d];
+  }
+  if (ncell > (1LL << 28)) return set_err(NDSP_ERR_CELLS, "link: too many cells");
+  g.ncell = ncell;
+  if (ncell + 1 > c->ncell_cap) {
+    if (a.cstart) hipFree(a.cstart);
+    if (a.cfill) hipFree(a.cfill);
+    if (c->bsum) hipFree(c->bsum);
+    long long cap = (ncell + 1) * 5 / 4 + 1024;
+    HIPCHK(hipMalloc(&a.cstart, sizeof(int) * cap));
+    HIPCHK(hipMalloc(&a.cfill, sizeof(int) * cap));
+    HIPCHK(hipMalloc(&c->bsum, sizeof(int) * (cap / (BS * 4) + 2)));
+    c->ncell_cap = cap;
+  }
+  c->g = g;
+  {
+    TScope t(c, "cellsort");
+    HIPCHK(hipMemsetAsync(a.cfill, 0, sizeof(int) * (ncell + 1), c->st));
+    hipLaunchKernelGGL(k_cellid, dim3(nblk(n)), dim3(BS), 0, c->st, a, g);
+    const long long m = ncell + 1;
+    const int nbs = (int)((m + BS * 4 - 1) / (BS * 4));
+    hipLaunchKernelGGL(k_scan_local, dim3(nbs), dim3(BS), 0, c->st, (const int*)a.cfill, a.cstart, c->bsum, m);
+    hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(BS), 0, c->st, c->bsum, nbs, (int*)nullptr);
+    hipLaunchKernelGGL(k_scan_add, dim3(nblk(m)), dim3(BS), 0, c->st, a.cstart, (const int*)c->bsum, m);
+    HIPCHK(hipMemsetAsync(a.cfill, 0, sizeof(int) * (ncell + 1), c->st));
+    hipLaunchKernelGGL(k_scatter, dim3(nblk(n)), dim3(BS), 0, c->st, a);
+    hipLaunchKernelGGL(k_cellsort, dim3(nblk(ncell)), dim3(BS), 0, c->st, a, ncell);
+  }
+  {
+    TScope t(c, "pack_density");
+    if (c->prec == NDSP_PREC_FP64)
+      hipLaunchKernelGGL(k_pack_density<double>, dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p);
+    else
+      hipLaunchKernelGGL(k_pack_density<float>, dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p);
+  }
+  HIPCHK(hipGetLastError());
+  c->grid_valid = true;
+  return 0;
+}
+
+static int do_density(ndsp_ctx* c) {
+  KArgs& a = c->a;
+  const int n = c->n;
+  hipLaunchKernelGGL(k_init_density_scalars, dim3(1), dim3(1), 0, c->st, a.sc, c->bconstmax);
+  bool reflect = false;
+  for (int d = 0; d < c->p.ndim; d++) if (c->ibound[d] == 2) reflect = true;
+  int its = 1, nlist = n;
+  const int* list = nullptr;
+  int* next = c->list_a;
+  int* other = c->list_b;
+  TScope t(c, "density");
+  while (true) {
+    const int sym = (nlist == n);
+    const int persistent = (!reflect && !sym) ? 1 : 0;
+    dispatch_ndim(c->p.ndim, [&](auto nd) {
+      constexpr int ND = decltype(nd)::value;
+      if (c->prec == NDSP_PREC_FP64)
+        hipLaunchKernelGGL((k_nr_pass<ND, double>), dim3(nblk(nlist)), dim3(BS), 0, c->st, a, c->p, c->g, list,
+                           nlist, its, sym, persistent, next);
+      else
+        hipLaunchKernelGGL((k_nr_pass<ND, float>), dim3(nblk(nlist)), dim3(BS), 0, c->st, a, c->p, c->g, list,
+                           nlist, its, sym, persistent, next);
+      return 0;
+    });
+    HIPCHK(hipGetLastError());
+    if (persistent) break;
+    DScal hs;
+    if (int e = sync_scalars(c, hs)) return e;
+    if (hs.err || hs.ncalc_next == 0 || its > c->p.maxits) break;
+    const int relink = reflect && hs.relink;
+    nlist = hs.ncalc_next;
+    list = next;
+    std::swap(next, other);
+    its++;
+    hipLaunchKernelGGL(k_next_pass_scalars, dim3(1), dim3(1), 0, c->st, a.sc, relink);
+    hipLaunchKernelGGL(k_hmax, dim3(std::min(1024u, nblk(n))), dim3(BS), 0, c->st, a, a.sc, relink);
+    if (nlist == n || relink)
+      hipLaunchKernelGGL(k_refresh_hinv, dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p, relink);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+static int do_rates(ndsp_ctx* c) {
+  KArgs& a = c->a;
+  const int n = c->n;
+  {
+    TScope t(c, "pack_force");
+    if (c->prec == NDSP_PREC_FP64)
+      hipLaunchKernelGGL(k_pack_force<double>, dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p);
+    else
+      hipLaunchKernelGGL(k_pack_force<float>, dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p);
+  }
+  hipLaunchKernelGGL(k_init_force_scalars, dim3(1), dim3(1), 0, c->st, a.sc);
+  {
+    TScope t(c, "force");
+    dispatch_ndim(c->p.ndim, [&](auto nd) {
+      constexpr int ND = decltype(nd)::value;
+      if (c->prec == NDSP_PREC_FP64)
+        hipLaunchKernelGGL((k_force<ND, double>), dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p, c->g);
+      else
+        hipLaunchKernelGGL((k_force<ND, float>), dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p, c->g);
+      return 0;
+    });
+  }
+  {
+    TScope t(c, "finalize");
+    hipLaunchKernelGGL(k_finalize, dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p);
+    hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, c->st, a.sc, c->p);
+  }
+  HIPCHK(hipGetLastError());
+  c->have_rates = true;
+  return 0;
+}
+
+static void fill_outputs(ndsp_ctx* c, const DScal& h, ndsp_stats* stats, ndsp_dt* dt) {
+  if (stats) {
+    stats->itsdensity = h.itsmax;
+    stats->nmin = h.nmin;
+    stats->nmax = h.nmax;
+    stats->ncalc_mean = (double)h.ncalc_total / (double)c->n;
+    stats->pairs_density = (int64_t)h.pairs_density;
+    stats->pairs_force = (int64_t)(h.pairs_force / 2);
+  }
+  if (dt) {
+    dt->dt = h.dt;
+    dt->dtcourant = h.dtcourant;
+    dt->dtforce = h.dtforce;
+    dt->dtdrag = h.dtdrag;
+    dt->vsig2max = (c->p.imhd && c->p.idivbzero >= 2) ? h.vsigmax * h.vsigmax : 0.0;
+    dt->hhmax = h.hmax_nr;
+    dt->stressmax = h.stressmax;
+    dt->time = c->time;
+  }
+}
+
+static int reorder(ndsp_ctx* c) {
+  KArgs& a = c->a;
+  const int n = c->n;
+  TScope t(c, "reorder");
+  for (int fi = 0; fi < F_NPERSIST; fi++) {
+    hipLaunchKernelGGL(k_gather_d, dim3(nblk(n)), dim3(BS), 0, c->st, (const double*)a.f[fi], c->scratch_d,
+                       (const int*)a.sidx, n);
+    std::swap(a.f[fi], c->scratch_d);
+  }
+  int** ia[3] = {&a.uid, &a.numneigh, &a.nits};
+  for (int q = 0; q < 3; q++) {
+    hipLaunchKernelGGL(k_gather_i, dim3(nblk(n)), dim3(BS), 0, c->st, (const int*)*ia[q], c->scratch_i,
+                       (const int*)a.sidx, n);
+    std::swap(*ia[q], c->scratch_i);
+  }
+  HIPCHK(hipGetLastError());
+  c->grid_valid = false;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+const char* ndsp_last_error(void) { return g_err.c_str(); }
+
+int ndsp_create(const int32_t* iopt, const double* dopt, int64_t npart, int32_t precision, ndsp_ctx** out) {
+  if (!iopt || !dopt || !out || npart < 1 || npart > 1000000000LL)
+    return set_err(NDSP_ERR_BAD_ARG, "ndsp_create: bad arguments");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(NDSP_ERR_NO_DEVICE, "no HIP device");
+  Phys p{};
+  p.ikernel = iopt[0]; p.iener = iopt[1]; p.imhd = iopt[2];
+  const int imagforce = iopt[3];
+  p.idivbzero = iopt[4];
+  const int iav = iopt[5];
+  for (int k = 0; k < 3; k++) p.iavlim[k] = iopt[6 + k];
+  p.idust = iopt[9];
+  const int idrag = iopt[10];
+  int ib[3] = {iopt[11], iopt[12], iopt[13]};
+  p.maxits = iopt[14];
+  const int idustevol = iopt[15];
+  p.ndim = iopt[16];
+  p.hfact = dopt[0]; p.tolh = dopt[1]; p.gamma = dopt[2]; p.polyk = dopt[3]; p.Kdrag = dopt[4];
+  p.alphamin = dopt[5]; p.alphaumin = dopt[6]; p.alphabmin = dopt[7]; p.beta = dopt[8];
+  p.avdecayconst = dopt[9]; p.psidecayfact = dopt[10]; p.C_cour = dopt[11]; p.C_force = dopt[12];
+  p.psep = dopt[19];
+  std::string bad;
+  if (p.ndim < 1 || p.ndim > 3) bad = "ndim must be 1, 2 or 3";
+  else if (p.ikernel != 0 && p.ikernel != 3) bad = "only ikernel 0 (cubic) and 3 (quintic)";
+  else if (p.iener != 0 && p.iener != 2) bad = "only iener 0 and 2";
+  else if (p.imhd != 0 && p.imhd != 11) bad = "only imhd 0 and 11";
+  else if (p.imhd && imagforce != 2) bad = "only imagforce 2";
+  else if (p.idivbzero != 0 && p.idivbzero != 2) bad = "only idivbzero 0 and 2";
+  else if (iav != 2) bad = "only iav 2";
+  else if (p.iavlim[0] < 0 || p.iavlim[0] > 2) bad = "iavlim(1) must be 0, 1 or 2";
+  else if (p.iavlim[1] < 0 || p.iavlim[1] > 1) bad = "iavlim(2) must be 0 or 1";
+  else if (!(p.iavlim[2] == 0 || p.iavlim[2] == 1 || p.iavlim[2] == 3)) bad = "iavlim(3) must be 0, 1 or 3";
+  else if (p.idust != 0 && p.idust != 4) bad = "only idust 0 and 4";
+  else if (p.idust == 4 && (idrag != 2 || idustevol != 0)) bad = "dust: only idrag_nature 2, idustevol 0";
+  else if (p.idust == 4 && p.imhd) bad = "dust with MHD not supported";
+  else if (dopt[24] != 0.0) bad = "damp must be 0";
+  else if (p.maxits < 0) bad = "maxdensits must be >= 0";
+  else if (precision != NDSP_PREC_MIXED && precision != NDSP_PREC_FP64) bad = "unknown precision";
+  int nrefl = 0;
+  for (int d = 0; d < p.ndim && bad.empty(); d++) {
+    if (!(ib[d] == 0 || ib[d] == 2 || ib[d] == 3)) bad = "ibound must be 0, 2 or 3";
+    if (ib[d] == 2) nrefl++;
+    if (ib[d] >= 2 && !(dopt[16 + d] > dopt[13 + d])) bad = "xmax must exceed xmin";
+  }
+  if (nrefl > 1 && bad.empty()) bad = "reflecting walls in at most one dimension";
+  if (!bad.empty()) return set_err(NDSP_ERR_BAD_OPTION, bad);
+  p.dndim = 1.0 / p.ndim;
+  p.boxmin = DBL_MAX;
+  for (int d = 0; d < p.ndim; d++)
+    if (ib[d] >= 2) p.boxmin = std::min(p.boxmin, dopt[16 + d] - dopt[13 + d]);
+  p.avfact = std::fabs(p.gamma - 1.) > 1.e-3 ? std::log(4.) / (std::log((p.gamma + 1.) / (p.gamma - 1.))) : 1.0;
+  std::vector<double> tab;
+  double dq2 = 0.0;
+  make_tables(p.ikernel, p.ndim, tab, p.radkern, dq2);
+  p.radkern2 = p.radkern * p.radkern;
+  p.dq2table = dq2;
+  p.ddq2table = 1.0 / dq2;
+
+  ndsp_ctx* c = new ndsp_ctx();
+  c->p = p;
+  c->prec = precision;
+  c->n = (int)npart;
+  c->bconstmax = std::max(std::max(dopt[20], dopt[21]), dopt[22]);
+  for (int d = 0; d < 3; d++) {
+    c->ibound[d] = d < p.ndim ? ib[d] : 0;
+    c->xmin[d] = dopt[13 + d];
+    c->xmax[d] = dopt[16 + d];
+    c->g.bound[d] = c->ibound[d];
+    c->g.xmin[d] = c->xmin[d];
+    c->g.xmax[d] = c->xmax[d];
+    c->g.nc[d] = 1;
+  }
+  c->g.ndim = p.ndim;
+  c->dt = dopt[23];
+  c->time = 0.0;
+  auto cleanup_fail = [&](int e) { ndsp_destroy(c); return e; };
+  if (hipStreamCreate(&c->st) != hipSuccess) return cleanup_fail(set_err(NDSP_ERR_HIP, "stream"));
+  KArgs& a = c->a;
+  memset(&a, 0, sizeof(a));
+  const size_t n = (size_t)npart;
+  for (int fi = 0; fi < F_NFIELDS; fi++) {
+    if (hipMalloc(&a.f[fi], sizeof(double) * n) != hipSuccess) return cleanup_fail(set_err(NDSP_ERR_ALLOC, "hipMalloc fields"));
+    hipMemset(a.f[fi], 0, sizeof(double) * n);
+  }
+  bool ok = true;
+  ok &= hipMalloc(&c->scratch_d, sizeof(double) * n) == hipSuccess;
+  ok &= hipMalloc(&c->scratch_i, sizeof(int) * n) == hipSuccess;
+  ok &= hipMalloc(&a.numneigh, sizeof(int) * n) == hipSuccess;
+  ok &= hipMalloc(&a.nits, sizeof(int) * n) == hipSuccess;
+  ok &= hipMalloc(&a.uid, sizeof(int) * n) == hipSuccess;
+  ok &= hipMalloc(&a.cellid, sizeof(int) * n) == hipSuccess;
+  ok &= hipMalloc(&a.sidx, sizeof(int) * n) == hipSuccess;
+  ok &= hipMalloc(&c->list_a, sizeof(int) * n) == hipSuccess;
+  ok &= hipMalloc(&c->list_b, sizeof(int) * n) == hipSuccess;
+  ok &= hipMalloc(&a.P4, sizeof(double4) * n) == hipSuccess;
+  ok &= hipMalloc(&a.dxb, sizeof(double) * n) == hipSuccess;
+  const size_t v4 = sizeof(double4) * n, v1 = sizeof(double) * n;
+  ok &= hipMalloc(&a.D4, v4) == hipSuccess;
+  ok &= hipMalloc(&a.De, v1) == hipSuccess;
+  ok &= hipMalloc(&a.G0, v4) == hipSuccess;
+  ok &= hipMalloc(&a.G1, v4) == hipSuccess;
+  ok &= hipMalloc(&a.G2, v4) == hipSuccess;
+  ok &= hipMalloc(&a.G3, v4) == hipSuccess;
+  ok &= hipMalloc(&a.G4, v4) == hipSuccess;
+  ok &= hipMalloc(&a.G5, v1) == hipSuccess;
+  ok &= hipMalloc((void**)&a.sc, sizeof(DScal)) == hipSuccess;
+  float* tabf = nullptr;
+  double* tabd = nullptr;
+  ok &= hipMalloc(&tabf, sizeof(float) * tab.size()) == hipSuccess;
+  ok &= hipMalloc(&tabd, sizeof(double) * tab.size()) == hipSuccess;
+  c->nb_part = 1024;
+  ok &= hipMalloc(&c->part, sizeof(double) * 32 * c->nb_part) == hipSuccess;
+  ok &= hipMalloc(&c->cons_out, sizeof(double) * 32) == hipSuccess;
+  if (!ok) return cleanup_fail(set_err(NDSP_ERR_ALLOC, "hipMalloc buffers"));
+  a.tabf = tabf;
+  a.tabd = tabd;
+  a.n = c->n;
+  std::vector<float> tf(tab.begin(), tab.end());
+  hipMemcpy(tabf, tf.data(), sizeof(float) * tf.size(), hipMemcpyHostToDevice);
+  hipMemcpy(tabd, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice);
+  DScal hs{};
+  hs.dt = c->dt;
+  hs.dtdrag = DBL_MAX;
+  hs.dtcourant = 1.0e6;
+  hipMemcpy(a.sc, &hs, sizeof(DScal), hipMemcpyHostToDevice);
+  hipMemset(a.numneigh, 0, sizeof(int) * n);
+  hipMemset(a.nits, 0, sizeof(int) * n);
+  if (hipDeviceSynchronize() != hipSuccess) return cleanup_fail(set_err(NDSP_ERR_HIP, "init sync"));
+  *out = c;
+  return 0;
+}
+
+int ndsp_destroy(ndsp_ctx* c) {
+  if (!c) return 0;
+  if (c->st) hipStreamSynchronize(c->st);
+  KArgs& a = c->a;
+  for (int fi = 0; fi < F_NFIELDS; fi++) if (a.f[fi]) hipFree(a.f[fi]);
+  void* ptrs[] = {c->scratch_d, c->scratch_i, a.numneigh, a.nits, a.uid, a.cellid, a.sidx, a.P4, a.dxb,
+                  a.D4, a.De, a.G0, a.G1, a.G2, a.G3, a.G4, a.G5, a.sc, (void*)a.tabf, (void*)a.tabd,
+                  c->part, c->cons_out, a.cstart, a.cfill, c->bsum, c->list_a, c->list_b};
+  for (void* q : ptrs) if (q) hipFree(q);
+  for (auto e : c->ev0) hipEventDestroy(e);
+  for (auto e : c->ev1) hipEventDestroy(e);
+  if (c->st) hipStreamDestroy(c->st);
+  delete c;
+  return 0;
+}
+
+int ndsp_upload(ndsp_ctx* c, const double* x, const double* vel, const double* pmass, const double* hh,
+                const double* en, const double* Bevol, const double* alpha, const double* psi,
+                const double* dustevol, const int32_t* itype) {
+  if (!c || !x || !vel || !pmass || !hh || !en || !alpha || !psi) return set_err(NDSP_ERR_BAD_ARG, "upload: null array");
+  const int n = c->n, nd = c->p.ndim;
+  std::vector<double> buf(n);
+  auto put = [&](int fi) { return hipMemcpyAsync(c->a.f[fi], buf.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->st); };
+  for (int i = 0; i < n; i++) {
+    if (itype && itype[i] != 0) return set_err(NDSP_ERR_BAD_OPTION, "only gas particles (itype 0) are supported");
+    if (!(hh[i] > 0.0)) return set_err(NDSP_ERR_H_NONPOSITIVE, "upload: h <= 0");
+  }
+  for (int d = 0; d < 3; d++) {
+    for (int i = 0; i < n; i++) buf[i] = d < nd ? x[(size_t)i * nd + d] : 0.0;
+    HIPCHK(put(F_X + d));
+    HIPCHK(hipStreamSynchronize(c->st));
+  }
+  const int vf[3] = {F_VX, F_VY, F_VZ}, bf[3] = {F_BX, F_BY, F_BZ}, af[3] = {F_A0, F_A1, F_A2};
+  for (int d = 0; d < 3; d++) {
+    for (int i = 0; i < n; i++) buf[i] = vel[(size_t)i * 3 + d];
+    HIPCHK(put(vf[d])); HIPCHK(hipStreamSynchronize(c->st));
+    for (int i = 0; i < n; i++) buf[i] = (c->p.imhd && Bevol) ? Bevol[(size_t)i * 3 + d] : 0.0;
+    HIPCHK(put(bf[d])); HIPCHK(hipStreamSynchronize(c->st));
+    for (int i = 0; i < n; i++) buf[i] = alpha[(size_t)i * 3 + d];
+    HIPCHK(put(af[d])); HIPCHK(hipStreamSynchronize(c->st));
+  }
+  const double* one[4] = {pmass, hh, en, psi};
+  const int onef[4] = {F_M, F_H, F_EN, F_PSI};
+  for (int q = 0; q < 4; q++) {
+    for (int i = 0; i < n; i++) buf[i] = one[q][i];
+    HIPCHK(put(onef[q])); HIPCHK(hipStreamSynchronize(c->st));
+  }
+  for (int i = 0; i < n; i++) buf[i] = (c->p.idust && dustevol) ? dustevol[i] : 0.0;
+  HIPCHK(put(F_EPS)); HIPCHK(hipStreamSynchronize(c->st));
+  // dustfrac starts equal to the evolved dust fraction
+  HIPCHK(hipMemcpy(c->a.f[F_DUSTFRAC], c->a.f[F_EPS], sizeof(double) * n, hipMemcpyDeviceToDevice));
+  std::vector<int> ids(n);
+  for (int i = 0; i < n; i++) ids[i] = i;
+  HIPCHK(hipMemcpy(c->a.uid, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  // derived fields and rates start at zero
+  for (int fi = F_RHO; fi < F_NFIELDS; fi++)
+    if (fi != F_DUSTFRAC) HIPCHK(hipMemsetAsync(c->a.f[fi], 0, sizeof(double) * n, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->grid_valid = false;
+  c->have_rates = false;
+  c->nsteps = 0;
+  return 0;
+}
+
+static const struct { const char* name; int f0; int w; } kFieldMap[] = {
+    {"vel", F_VX, 3}, {"pmass", F_M, 1}, {"hh", F_H, 1}, {"en", F_EN, 1}, {"Bevol", F_BX, 3},
+    {"alpha", F_A0, 3}, {"psi", F_PSI, 1}, {"dustevol", F_EPS, 1}, {"rho", F_RHO, 1},
+    {"gradh", F_GRADH, 1}, {"gradgradh", F_GRADGRADH, 1}, {"drhodt", F_DRHODT, 1},
+    {"dhdt", F_DHDT, 1}, {"dens", F_DENS, 1}, {"pr", F_PR, 1}, {"spsound", F_CS, 1},
+    {"uu", F_UU, 1}, {"Bfield", F_BX, 3}, {"force", F_FX, 3}, {"dudt", F_DUDT, 1},
+    {"dendt", F_DENDT, 1}, {"dBevoldt", F_DBX, 3}, {"daldt", F_DA0, 3}, {"dpsidt", F_DPSIDT, 1},
+    {"divB", F_DIVB, 1}, {"curlB", F_CBX, 3}, {"gradpsi", F_GPX, 3}, {"dustfrac", F_DUSTFRAC, 1},
+    {"rhogas", F_RHOGAS, 1}, {"rhodust", F_RHODUST, 1}, {"ddustevoldt", F_DEPS, 1},
+    {"deltav", F_DVX, 3}, {"del2u", F_DEL2U, 1}};
+
+int ndsp_download(ndsp_ctx* c, const char* name, double* out, int32_t* width) {
+  if (!c || !name || !out) return set_err(NDSP_ERR_BAD_ARG, "download: null argument");
+  const int n = c->n;
+  std::vector<int> uid(n);
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipMemcpy(uid.data(), c->a.uid, sizeof(int) * n, hipMemcpyDeviceToHost));
+  std::vector<double> buf(n);
+  if (!strcmp(name, "x")) {
+    const int nd = c->p.ndim;
+    for (int d = 0; d < nd; d++) {
+      HIPCHK(hipMemcpy(buf.data(), c->a.f[F_X + d], sizeof(double) * n, hipMemcpyDeviceToHost));
+      for (int s = 0; s < n; s++) out[(size_t)uid[s] * nd + d] = buf[s];
+    }
+    if (width) *width = nd;
+    return 0;
+  }
+  if (!strcmp(name, "numneigh") || !strcmp(name, "nits")) {
+    std::vector<int> ib(n);
+    HIPCHK(hipMemcpy(ib.data(), !strcmp(name, "nits") ? c->a.nits : c->a.numneigh, sizeof(int) * n,
+                     hipMemcpyDeviceToHost));
+    for (int s = 0; s < n; s++) out[uid[s]] = ib[s];
+    if (width) *width = 1;
+    return 0;
+  }
+  for (const auto& e : kFieldMap) {
+    if (strcmp(name, e.name)) continue;
+    for (int q = 0; q < e.w; q++) {
+      HIPCHK(hipMemcpy(buf.data(), c->a.f[e.f0 + q], sizeof(double) * n, hipMemcpyDeviceToHost));
+      for (int s = 0; s < n; s++) out[(size_t)uid[s] * e.w + q] = buf[s];
+    }
+    if (width) *width = e.w;
+    return 0;
+  }
+  return set_err(NDSP_ERR_BAD_ARG, std::string("download: unknown field ") + name);
+}
+
+int ndsp_set_linklist(ndsp_ctx* c) {
+  if (!c) return set_err(NDSP_ERR_BAD_ARG, "null context");
+  return do_linklist(c, 0);
+}
+
+int ndsp_iterate_density(ndsp_ctx* c, ndsp_stats* stats) {
+  if (!c) return set_err(NDSP_ERR_BAD_ARG, "null context");
+  if (!c->grid_valid) return set_err(NDSP_ERR_BAD_ARG, "iterate_density: call set_linklist first");
+  if (int e = do_density(c)) return e;
+  DScal hs;
+  if (int e = sync_scalars(c, hs)) return e;
+  timing_collect(c);
+  fill_outputs(c, hs, stats, nullptr);
+  return check_err(c, hs);
+}
+
+int ndsp_get_rates(ndsp_ctx* c, ndsp_dt* dt) {
+  if (!c) return set_err(NDSP_ERR_BAD_ARG, "null context");
+  if (!c->grid_valid) return set_err(NDSP_ERR_BAD_ARG, "get_rates: call set_linklist and iterate_density first");
+  if (int e = do_rates(c)) return e;
+  DScal hs;
+  if (int e = sync_scalars(c, hs)) return e;
+  timing_collect(c);
+  fill_outputs(c, hs, nullptr, dt);
+  return check_err(c, hs);
+}
+
+int ndsp_derivs(ndsp_ctx* c, ndsp_stats* stats, ndsp_dt* dt) {
+  if (!c) return set_err(NDSP_ERR_BAD_ARG, "null context");
+  if (int e = do_linklist(c, 0)) return e;
+  if (int e = do_density(c)) return e;
+  if (int e = do_rates(c)) return e;
+  DScal hs;
+  if (int e = sync_scalars(c, hs)) return e;
+  timing_collect(c);
+  fill_outputs(c, hs, stats, dt);
+  return check_err(c, hs);
+}
+
+int ndsp_set_dt(ndsp_ctx* c, double dt, double time) {
+  if (!c) return set_err(NDSP_ERR_BAD_ARG, "null context");
+  c->dt = dt;
+  c->time = time;
+  HIPCHK(hipMemcpy(&c->a.sc->dt, &dt, sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int ndsp_step(ndsp_ctx* c, ndsp_stats* stats, ndsp_dt* dtout) {
+  if (!c) return set_err(NDSP_ERR_BAD_ARG, "null context");
+  if (!c->have_rates) {
+    // the reference evaluates derivs once before the first step
+    // (primitive2conservative, src/conservative2primitive.f90:799)
+    if (int e = ndsp_derivs(c, nullptr, nullptr)) return e;
+  }
+  if (c->grid_valid && c->reorder_every > 0 && c->nsteps % c->reorder_every == 0) {
+    if (int e = reorder(c)) return e;
+  }
+  const double dt = c->dt;
+  {
+    TScope t(c, "predict");
+    hipLaunchKernelGGL(k_predict, dim3(nblk(c->n)), dim3(BS), 0, c->st, c->a, c->p, dt);
+  }
+  if (int e = do_linklist(c, 1)) return e;
+  if (int e = do_density(c)) return e;
+  if (int e = do_rates(c)) return e;
+  {
+    TScope t(c, "correct");
+    hipLaunchKernelGGL(k_correct, dim3(nblk(c->n)), dim3(BS), 0, c->st, c->a, c->p, dt);
+    hipLaunchKernelGGL(k_boundary, dim3(nblk(c->n)), dim3(BS), 0, c->st, c->a, c->p, c->g, 1);
+  }
+  HIPCHK(hipGetLastError());
+  DScal hs;
+  if (int e = sync_scalars(c, hs)) return e;
+  timing_collect(c);
+  c->time += dt;
+  c->dt = hs.dt;
+  c->nsteps++;
+  fill_outputs(c, hs, stats, dtout);
+  return check_err(c, hs);
+}
+
+int ndsp_conserved(ndsp_ctx* c, double* out) {
+  if (!c || !out) return set_err(NDSP_ERR_BAD_ARG, "null argument");
+  const int nb = std::min(c->nb_part, (int)nblk(c->n));
+  hipLaunchKernelGGL(k_conserved, dim3(nb), dim3(BS), 0, c->st, c->a, c->p, c->part);
+  hipLaunchKernelGGL(k_conserved_final, dim3(1), dim3(1), 0, c->st, (const double*)c->part, nb, c->cons_out, c->n);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, c->cons_out, sizeof(double) * NCONS, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int ndsp_neighbours(ndsp_ctx* c, int32_t sym, int32_t* pairs, int64_t cap, int64_t* npairs) {
+  if (!c || !npairs) return set_err(NDSP_ERR_BAD_ARG, "null argument");
+  if (!c->grid_valid || !c->have_rates) return set_err(NDSP_ERR_BAD_ARG, "neighbours: call derivs first");
+  const int n = c->n;
+  int* counts = nullptr;
+  int* offs = nullptr;
+  int* dpairs = nullptr;
+  HIPCHK(hipMalloc(&counts, sizeof(int) * n));
+  dispatch_ndim(c->p.ndim, [&](auto nd) {
+    constexpr int ND = decltype(nd)::value;
+    hipLaunchKernelGGL((k_nb<ND>), dim3(nblk(n)), dim3(BS), 0, c->st, c->a, c->p, c->g, (int)sym,
+                       (const int*)nullptr, counts);
+    return 0;
+  });
+  std::vector<int> hc(n), ho(n);
+  HIPCHK(hipMemcpy(hc.data(), counts, sizeof(int) * n, hipMemcpyDeviceToHost));
+  long long tot = 0;
+  for (int k = 0; k < n; k++) { ho[k] = (int)tot; tot += hc[k]; }
+  *npairs = tot;
+  if (tot > cap || !pairs) {
+    hipFree(counts);
+    return set_err(NDSP_ERR_BAD_ARG, "neighbours: buffer too small");
+  }
+  HIPCHK(hipMalloc(&offs, sizeof(int) * n));
+  HIPCHK(hipMalloc(&dpairs, sizeof(int) * 3 * std::max(1LL, tot)));
+  HIPCHK(hipMemcpy(offs, ho.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  dispatch_ndim(c->p.ndim, [&](auto nd) {
+    constexpr int ND = decltype(nd)::value;
+    hipLaunchKernelGGL((k_nb<ND>), dim3(nblk(n)), dim3(BS), 0, c->st, c->a, c->p, c->g, (int)sym,
+                       (const int*)offs, dpairs);
+    return 0;
+  });
+  HIPCHK(hipMemcpy(pairs, dpairs, sizeof(int) * 3 * tot, hipMemcpyDeviceToHost));
+  hipFree(counts);
+  hipFree(offs);
+  hipFree(dpairs);
+  return 0;
+}
+
+int ndsp_set_timing(ndsp_ctx* c, int32_t on) {
+  if (!c) return set_err(NDSP_ERR_BAD_ARG, "null context");
+  c->timing = on != 0;
+  for (auto& v : c->tms) v = 0.0;
+  for (auto& v : c->tcount) v = 0;
+  return 0;
+}
+
+int ndsp_kernel_times(ndsp_ctx* c, double* ms, int32_t cap, int32_t* nout, char* names, int32_t names_cap) {
+  if (!c || !nout) return set_err(NDSP_ERR_BAD_ARG, "null argument");
+  int n = (int)c->tnames.size();
+  *nout = n;
+  std::string all;
+  for (int k = 0; k < n; k++) {
+    if (ms && k < cap) ms[k] = c->tcount[k] ? c->tms[k] / c->tcount[k] : 0.0;
+    if (k) all += ",";
+    all += c->tnames[k];
+  }
+  if (names && names_cap > 0) {
+    strncpy(names, all.c_str(), names_cap - 1);
+    names[names_cap - 1] = 0;
+  }
+  return 0;
+}
+
+}  // extern "C"

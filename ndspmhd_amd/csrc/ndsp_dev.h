@@ -1,0 +1,164 @@
+// Device-side building blocks of the MI355X hot path: cell grid and periodic /
+// reflecting image arithmetic, kernel-table interpolation, compensated sums.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define NDSP_IKERN 4000
+
+// Cell grid.  Dimensions with walls (periodic or reflecting) are tiled exactly
+// by nc cells of width w = (xmax - xmin)/nc >= radkern*hmax, so a periodic or
+// mirror image of a cell is again a whole cell; open dimensions tile the
+// particle bounding box.
+struct Grid {
+  int ndim;
+  int nc[3];
+  int bound[3];      // 0 open, 2 reflecting, 3 periodic
+  double lo[3];      // grid origin
+  double w[3];       // cell width
+  double inv_w[3];
+  double xmin[3], xmax[3];
+  long long ncell;
+};
+
+// Image of a cell at unwrapped coordinate c along one dimension.
+//   kind 0: inside, 1: periodic image left of xmin (source near xmax),
+//   2: periodic image right of xmax (source near xmin),
+//   3: mirror about xmin, 4: mirror about xmax.
+// Returns the source cell, or -1 if the unwrapped cell is unreachable.
+__device__ __forceinline__ int cell_image(const Grid& g, int d, int c, int& kind) {
+  const int n = g.nc[d];
+  if (c >= 0 && c < n) { kind = 0; return c; }
+  if (g.bound[d] == 3) {
+    if (c < 0 && c >= -n) { kind = 1; return c + n; }
+    if (c >= n && c < 2 * n) { kind = 2; return c - n; }
+    return -1;
+  }
+  if (g.bound[d] == 2) {
+    if (c < 0 && c >= -n) { kind = 3; return -1 - c; }
+    if (c >= n && c < 2 * n) { kind = 4; return 2 * n - 1 - c; }
+    return -1;
+  }
+  return -1;
+}
+
+// Coordinate of an image along one dimension, with the reference's ghost
+// arithmetic (src/ghostND_mhd.f90:180-200: xnew = xperbound + dxshift for
+// periodic, xbound - dxshift for reflecting, dxshift = x - xbound), and the
+// existence test of a ghost copy (0 < distance to the wall < dxbound,
+// src/ghostND_mhd.f90:172).  dxb is the reflecting dxbound radkern*h_j at
+// ghost-creation time; periodic images of true neighbours always satisfy the
+// upper bound, only the strict positivity is tested.
+__device__ __forceinline__ bool image_coord(const Grid& g, int d, int kind, double xj,
+                                            double dxb, double& ximg) {
+#pragma clang fp contract(off)
+  switch (kind) {
+    case 0: ximg = xj; return true;
+    case 1: {  // source near xmax, ghost at xmin + (x - xmax)
+      double dx = g.xmax[d] - xj;
+      ximg = g.xmin[d] + (xj - g.xmax[d]);
+      return dx > 0.0;
+    }
+    case 2: {  // source near xmin, ghost at xmax + (x - xmin)
+      double dx = xj - g.xmin[d];
+      ximg = g.xmax[d] + (xj - g.xmin[d]);
+      return dx > 0.0;
+    }
+    case 3: {  // mirror about xmin: xmin - (x - xmin)
+      double dx = xj - g.xmin[d];
+      ximg = g.xmin[d] - (xj - g.xmin[d]);
+      return dx > 0.0 && dx < dxb;
+    }
+    default: {  // mirror about xmax: xmax - (x - xmax)
+      double dx = g.xmax[d] - xj;
+      ximg = g.xmax[d] - (xj - g.xmax[d]);
+      return dx > 0.0 && dx < dxb;
+    }
+  }
+}
+
+// unwrapped cell range covering [x - rad, x + rad] along d (a small margin on
+// both sides keeps every true neighbour in range; the exact FP64 distance test
+// decides membership)
+__device__ __forceinline__ void cell_range(const Grid& g, int d, double x, double rad, int& lo,
+                                           int& hi) {
+  double a = (x - rad - g.lo[d]) * g.inv_w[d];
+  double b = (x + rad - g.lo[d]) * g.inv_w[d];
+  lo = (int)floor(a - 1.0e-9);
+  hi = (int)floor(b + 1.0e-9);
+  if (g.bound[d] == 0) {
+    if (lo < 0) lo = 0;
+    if (hi > g.nc[d] - 1) hi = g.nc[d] - 1;
+  } else {
+    if (lo < -g.nc[d]) lo = -g.nc[d];
+    if (hi > 2 * g.nc[d] - 1) hi = 2 * g.nc[d] - 1;
+  }
+}
+
+// FP64 squared separation with the reference's operation order and no FMA
+// contraction: rij2 = dot_product(dx,dx), left to right
+// (src/density_sums.f90:181, src/ratesND_mhd.f90:405)
+template <int NDIM>
+__device__ __forceinline__ double rij2_exact(const double* dx) {
+#pragma clang fp contract(off)
+  double r = dx[0] * dx[0];
+  if (NDIM >= 2) r = r + dx[1] * dx[1];
+  if (NDIM >= 3) r = r + dx[2] * dx[2];
+  return r;
+}
+
+__device__ __forceinline__ double mul_exact(double a, double b) {
+#pragma clang fp contract(off)
+  return a * b;
+}
+
+// compensated accumulator: FP32 uses the branch-free TwoSum error term
+// (Knuth), FP64 is a plain sum
+template <typename T>
+struct Acc {
+  T s, c;
+  __device__ __forceinline__ void zero() { s = T(0); c = T(0); }
+  __device__ __forceinline__ void add(T x) {
+    T t = s + x;
+    T bp = t - s;
+    c += (s - (t - bp)) + (x - bp);
+    s = t;
+  }
+  __device__ __forceinline__ double value() const { return (double)s + (double)c; }
+};
+template <>
+struct Acc<double> {
+  double s;
+  __device__ __forceinline__ void zero() { s = 0.0; }
+  __device__ __forceinline__ void add(double x) { s += x; }
+  __device__ __forceinline__ double value() const { return s; }
+};
+
+// linear interpolation in q^2 from a table held in LDS
+// (interpolate_kernels_dens, src/kernelND.f90:4599-4638)
+template <typename T>
+__device__ __forceinline__ T ktab(const T* __restrict__ tab, T q2, T ddq2, T dq2) {
+  T t = q2 * ddq2;
+  int index = (t < T(NDSP_IKERN + 1)) ? (int)t : NDSP_IKERN + 1;
+  int index1 = index + 1;
+  if (index > NDSP_IKERN) index = NDSP_IKERN;
+  if (index1 > NDSP_IKERN) index1 = NDSP_IKERN;
+  T dxx = q2 - T(index) * dq2;
+  T w0 = tab[index];
+  return w0 + (tab[index1] - w0) * ddq2 * dxx;
+}
+
+template <int N, typename T>
+__device__ __forceinline__ T pown_t(T x) {
+  if (N == 1) return x;
+  if (N == 2) return x * x;
+  return x * x * x;
+}
+
+// atomics on non-negative doubles through their IEEE bit patterns
+__device__ __forceinline__ void atomic_max_pos(double* a, double v) {
+  atomicMax((unsigned long long*)a, (unsigned long long)__double_as_longlong(v));
+}
+__device__ __forceinline__ void atomic_min_pos(double* a, double v) {
+  atomicMin((unsigned long long*)a, (unsigned long long)__double_as_longlong(v));
+}

@@ -1,0 +1,92 @@
+// Kernel-side data layout shared by ndsp.hip: field enumeration of the
+// particle state held in HBM (structure of arrays, FP64) and the argument
+// block every kernel receives by value.
+#pragma once
+#include "ndsp_dev.h"
+
+enum Field {
+  // evolved state
+  F_X, F_Y, F_Z, F_VX, F_VY, F_VZ, F_BX, F_BY, F_BZ, F_M, F_H, F_EN, F_PSI, F_EPS,
+  F_A0, F_A1, F_A2,
+  // derived by the density pass
+  F_RHO, F_GRADH, F_GRADGRADH, F_DRHODT, F_DHDT, F_DENS, F_PR, F_CS, F_UU,
+  F_DUSTFRAC, F_RHOGAS, F_RHODUST,
+  // rates
+  F_FX, F_FY, F_FZ, F_DUDT, F_DENDT, F_DBX, F_DBY, F_DBZ, F_DA0, F_DA1, F_DA2,
+  F_DPSIDT, F_DIVB, F_CBX, F_CBY, F_CBZ, F_GPX, F_GPY, F_GPZ, F_DEPS, F_DVX,
+  F_DVY, F_DVZ, F_DEL2U,
+  F_NPERSIST,  // fields above are carried by a reorder
+  // step in-copies (src/stepND_leapfrog_mhd.f90:70-100)
+  F_XIN = F_NPERSIST, F_YIN, F_ZIN, F_VXIN, F_VYIN, F_VZIN, F_BXIN, F_BYIN, F_BZIN,
+  F_HIN, F_ENIN, F_A0IN, F_A1IN, F_A2IN, F_PSIIN, F_EPSIN, F_FXIN, F_FYIN, F_FZIN,
+  F_DBXIN, F_DBYIN, F_DBZIN, F_DHDTIN, F_DENDTIN, F_DA0IN, F_DA1IN, F_DA2IN,
+  F_DPSIDTIN, F_DEPSIN,
+  F_HNR0,  // h at entry of the h-rho iteration (hhin of src/iterate_density.f90:95)
+  F_NFIELDS
+};
+
+// physics switches resolved once at context creation
+struct Phys {
+  int ndim, ikernel, iener, imhd, idivbzero, idust, iavlim[3], maxits;
+  double hfact, tolh, gamma, polyk, Kdrag, alphamin, alphaumin, alphabmin, beta;
+  double avdecayconst, psidecayfact, C_cour, C_force, psep, avfact, dndim;
+  double radkern, radkern2, dq2table, ddq2table;
+  double Bconstmax;
+  double boxmin;  // smallest wall-to-wall length (search radii must stay below it)
+};
+
+// device scalars (one per context); doubles touched by atomics hold
+// non-negative values
+struct DScal {
+  double hmax_bin;    // max h at binning (src/ghostND_mhd.f90:77)
+  double hmax_nr;     // max h after the h-rho iteration
+  double stressmax;   // src/ratesND_mhd.f90:231-245
+  double vsigmax;     // src/ratesND_mhd.f90:1475
+  double dtcourant;   // src/ratesND_mhd.f90:1481
+  double fhmax;       // src/ratesND_mhd.f90:814-816
+  double dtdrag;      // src/ratesND_mhd.f90:606
+  double dtforce;
+  double dt, time;
+  double hmax_cur;     // max current h (search radius of a symmetric pass)
+  double hhmax_ghost;  // hhmax of the last ghost set-up (relink test, iterate_density.f90:260)
+  double bbox_lo[3], bbox_hi[3];  // of the real particles (open dimensions)
+  unsigned long long pairs_density, pairs_force, ncalc_total;
+  int err;
+  int itsmax;
+  int nmin, nmax;
+  int ncalc_next;      // unconverged particles after a pass
+  int relink;
+};
+
+template <typename PT>
+struct V4 { typedef float4 type; };
+template <>
+struct V4<double> { typedef double4 type; };
+
+struct KArgs {
+  double* f[F_NFIELDS];
+  int* numneigh;
+  int* nits;
+  int* uid;           // caller index of each slot
+  // binning
+  int* cellid;        // per slot
+  int* cstart;        // ncell + 1
+  int* cfill;         // ncell
+  int* sidx;          // sorted position -> slot
+  // packed neighbour views in sorted order
+  double4* P4;        // x, y, z, 1/h
+  double* dxb;        // reflecting ghost extent radkern*h at ghost-creation time
+  void* D4;           // density: vx, vy, vz, m
+  void* De;           // density: dust fraction
+  void* G0;           // force: vx vy vz m
+  void* G1;           // force: rho pr cs gradh
+  void* G2;           // force: uu a0 a1 a2
+  void* G3;           // force: Bx By Bz psi
+  void* G4;           // force: h dens eps rhogas
+  void* G5;           // force: rhodust
+  // kernel tables (FP32 and FP64 copies of W, dW, d2W)
+  const float* tabf;  // 3 x (IKERN+1)
+  const double* tabd;
+  DScal* sc;
+  int n;
+};

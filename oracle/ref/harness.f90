@@ -48,6 +48,7 @@ program ndsp_ref_harness
  integer, allocatable :: listneigh(:),pg(:,:),ps(:,:)
  real, allocatable :: dustin(:),zeros(:),zeros3(:,:)
  real :: rij2,q2i,q2j,hi1,hi21,hj1
+ integer(8) :: count0,count1,count_rate
  real, dimension(ndim) :: dx
  character(len=256) :: infile,outfile
 
@@ -170,9 +171,13 @@ program ndsp_ref_harness
  drhodt = 0.
  dhdt = 0.
 !
-!--the hot path: one derivative evaluation
+!--the hot path: one derivative evaluation (wall time printed for the CPU
+!  baseline)
 !
+ call system_clock(count0,count_rate)
  call derivs
+ call system_clock(count1)
+ write(*,"(a,es16.8)") ' NDSP_DERIVS_SECONDS ',real(count1-count0)/real(count_rate)
 !
 !--results
 !

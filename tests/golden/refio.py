@@ -16,6 +16,7 @@ Output file:
 from __future__ import annotations
 
 import os
+import resource
 import subprocess
 import tempfile
 
@@ -70,6 +71,12 @@ def read_output(path, ndim, mode=1):
     return out
 
 
+def _unlimited_stack():
+    # the reference keeps large automatic arrays on the stack
+    # (src/allocateND.f90:62-100, src/iterate_density.f90:63-66)
+    resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+
 def run_reference(binary, opts, st, mode=1, timeout=600):
     """Run the compiled reference harness on (opts, st); returns its outputs."""
     with tempfile.TemporaryDirectory() as d:
@@ -77,7 +84,7 @@ def run_reference(binary, opts, st, mode=1, timeout=600):
         fout = os.path.join(d, "out.bin")
         write_input(fin, opts, st, mode)
         p = subprocess.run([os.path.abspath(binary), fin, fout], cwd=d, capture_output=True,
-                           text=True, timeout=timeout)
+                           text=True, timeout=timeout, preexec_fn=_unlimited_stack)
         if p.returncode != 0:
             raise RuntimeError("reference harness failed (%d):\n%s\n%s"
                                % (p.returncode, p.stdout[-4000:], p.stderr[-4000:]))

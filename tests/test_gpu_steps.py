@@ -1,0 +1,65 @@
+"""Leapfrog steps on the GPU against the CPU oracle (same inputs, same dt
+sequence): FP64 path to round-off; mixed path within 1e-4 after several
+steps (single-evaluation errors are below 1e-5, test_gpu_parity.py, and the
+trajectories then drift apart slowly through the positions)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(o, st, nsteps, prec):
+    import ndspmhd_amd as nd
+    import oracle
+    sim = nd.Simulation(o, st, precision=prec)
+    orc = oracle.Oracle(o, st)
+    sim.derivs()
+    orc.derivs()
+    for s in range(nsteps):
+        # same dt on both sides: the oracle's
+        sim.set_dt(orc.scalar("dt"), orc.scalar("time"))
+        orc.step()
+        sim.step()
+    return sim, orc
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-9), (0, 1e-4)])
+def test_mhd_box_steps(prec, tol):
+    from ndspmhd_amd.setups import periodic_mhd_box
+    o, st = periodic_mhd_box(10, seed=3, vel_seed=4)
+    rng = np.random.default_rng(5)
+    st["Bevol"] += 0.03 * rng.standard_normal(st["Bevol"].shape)
+    print("mhd box 10^3 steps prec", prec, flush=True)
+    sim, orc = _run(o, st, 5, prec)
+    for k in ("x", "vel", "Bevol", "en", "hh", "alpha", "psi", "rho", "force", "dBevoldt"):
+        a, b = sim.get(k), orc.get(k)
+        err = np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+        print(k, err, flush=True)
+        assert err <= (tol if k != "x" else tol * 1e-2), (k, err)
+    assert sim.dt_info["dt"] == pytest.approx(orc.scalar("dt"), rel=10 * tol)
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-9), (0, 1e-4)])
+def test_dust_box_2d_steps(prec, tol):
+    from ndspmhd_amd.setups import dusty_box
+    o, st = dusty_box(24, ndim=2, seed=9)
+    print("dust box 24^2 steps prec", prec, flush=True)
+    sim, orc = _run(o, st, 5, prec)
+    for k in ("x", "vel", "dustevol", "hh", "rho", "force", "ddustevoldt", "deltav"):
+        a, b = sim.get(k), orc.get(k)
+        err = np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+        print(k, err, flush=True)
+        assert err <= (tol if k != "x" else tol * 1e-2), (k, err)
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-9), (0, 1e-4)])
+def test_brio_wu_1d_steps(prec, tol):
+    from ndspmhd_amd.setups import brio_wu_1d
+    o, st = brio_wu_1d(400)
+    print("brio-wu 1d steps prec", prec, flush=True)
+    sim, orc = _run(o, st, 10, prec)
+    for k in ("x", "vel", "Bevol", "en", "hh", "rho", "force"):
+        a, b = sim.get(k), orc.get(k)
+        err = np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+        print(k, err, flush=True)
+        assert err <= (tol if k != "x" else tol * 1e-2), (k, err)
