@@ -148,6 +148,27 @@ int ndsp_set_timing(ndsp_ctx *ctx, int32_t on);
 int ndsp_kernel_times(ndsp_ctx *ctx, double *ms, int32_t cap, int32_t *n,
                       char *names, int32_t names_cap);
 
+/* ---- h-predictor MLP (no counterpart in the reference; its seed for the
+ * h-rho iteration is h = h_in + dt dh/dt, src/stepND_leapfrog_mhd.f90:153).
+ * A 6 -> 32 -> 32 (GELU) -> 1 network, BF16 weights, FP32 accumulation,
+ * predicts 1000 log(h_converged/h_predictor) from per-particle features; when
+ * enabled, ndsp_step seeds the iteration with it (the iteration still
+ * converges every particle to tolh) and, with train != 0, takes `iters` Adam
+ * steps per step on `batch` of that step's converged (features, h) pairs.
+ * Parameters: float[1313] = W1[32][6] b1[32] W2[32][32] b2[32] W3[32] b3. */
+int ndsp_mlp_config(ndsp_ctx *ctx, int32_t enable, int32_t train, double lr,
+                    int32_t batch, int32_t iters, int32_t seed_after_steps);
+int ndsp_mlp_set_params(ndsp_ctx *ctx, const float *params, int32_t n);
+int ndsp_mlp_get_params(ndsp_ctx *ctx, float *params, int32_t n, int32_t *adam_steps);
+/* forward of B feature rows X[B][6] with the BF16 weights */
+int ndsp_mlp_forward(ndsp_ctx *ctx, const float *X, int32_t B, float *y);
+/* gradient of sum (y - T)^2 / B over a caller batch (FP32 accumulation) */
+int ndsp_mlp_grad(ndsp_ctx *ctx, const float *X, const float *T, int32_t B, float *grad,
+                  double *loss);
+/* iters Adam steps on a caller batch; loss of the last step */
+int ndsp_mlp_train(ndsp_ctx *ctx, const float *X, const float *T, int32_t B, int32_t iters,
+                   double lr, double *loss);
+
 const char *ndsp_last_error(void);
 
 #ifdef __cplusplus

@@ -35,7 +35,9 @@ SYMBOLS = ("ndsp_create", "ndsp_destroy", "ndsp_upload", "ndsp_download",
            "ndsp_set_linklist", "ndsp_iterate_density", "ndsp_get_rates",
            "ndsp_derivs", "ndsp_step", "ndsp_set_dt", "ndsp_conserved",
            "ndsp_neighbours", "ndsp_set_timing", "ndsp_kernel_times",
-           "ndsp_last_error")
+           "ndsp_mlp_config", "ndsp_mlp_set_params", "ndsp_mlp_get_params",
+           "ndsp_mlp_forward", "ndsp_mlp_grad", "ndsp_mlp_train", "ndsp_last_error")
+MLP_NPARAM = 1313
 
 
 class NdspError(RuntimeError):
@@ -80,6 +82,14 @@ def load_library(path: str | None = None):
     L.ndsp_neighbours.argtypes = [vp, ctypes.c_int32, ip, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
     L.ndsp_set_timing.argtypes = [vp, ctypes.c_int32]
     L.ndsp_kernel_times.argtypes = [vp, dp, ctypes.c_int32, ip, ctypes.c_char_p, ctypes.c_int32]
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.ndsp_mlp_config.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
+                                  ctypes.c_int32, ctypes.c_int32]
+    L.ndsp_mlp_set_params.argtypes = [vp, fp, ctypes.c_int32]
+    L.ndsp_mlp_get_params.argtypes = [vp, fp, ctypes.c_int32, ip]
+    L.ndsp_mlp_forward.argtypes = [vp, fp, ctypes.c_int32, fp]
+    L.ndsp_mlp_grad.argtypes = [vp, fp, fp, ctypes.c_int32, fp, dp]
+    L.ndsp_mlp_train.argtypes = [vp, fp, fp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, dp]
     L.ndsp_last_error.restype = ctypes.c_char_p
     _lib = L
     return L
@@ -91,6 +101,10 @@ def _dp(a):
 
 def _ip(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
 class Simulation:
@@ -183,6 +197,43 @@ class Simulation:
         buf = np.empty(max(1, npairs.value) * 3, dtype=np.int32)
         self._check(L.ndsp_neighbours(self.h, int(sym), _ip(buf), npairs.value, ctypes.byref(npairs)))
         return buf[: 3 * npairs.value].reshape(-1, 3)
+
+    # --- h-predictor MLP ---
+    def mlp_config(self, enable=True, train=True, lr=1e-3, batch=16384, iters=4, seed_after=2):
+        self._check(load_library().ndsp_mlp_config(self.h, int(enable), int(train), float(lr),
+                                                   int(batch), int(iters), int(seed_after)))
+
+    def mlp_set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self._check(load_library().ndsp_mlp_set_params(self.h, _fp(p), p.size))
+
+    def mlp_get_params(self):
+        p = np.empty(MLP_NPARAM, dtype=np.float32)
+        k = ctypes.c_int32()
+        self._check(load_library().ndsp_mlp_get_params(self.h, _fp(p), p.size, ctypes.byref(k)))
+        return p, k.value
+
+    def mlp_forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.empty(X.shape[0], dtype=np.float32)
+        self._check(load_library().ndsp_mlp_forward(self.h, _fp(X), X.shape[0], _fp(y)))
+        return y
+
+    def mlp_grad(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.empty(MLP_NPARAM, dtype=np.float32)
+        loss = ctypes.c_double()
+        self._check(load_library().ndsp_mlp_grad(self.h, _fp(X), _fp(T), X.shape[0], _fp(g), ctypes.byref(loss)))
+        return loss.value, g
+
+    def mlp_train(self, X, T, iters, lr):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        loss = ctypes.c_double()
+        self._check(load_library().ndsp_mlp_train(self.h, _fp(X), _fp(T), X.shape[0], int(iters), float(lr),
+                                                  ctypes.byref(loss)))
+        return loss.value
 
     def set_timing(self, on: bool):
         self._check(load_library().ndsp_set_timing(self.h, int(on)))

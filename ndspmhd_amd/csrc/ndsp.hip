@@ -36,6 +36,7 @@
 
 #include "ndsp.h"
 #include "ndsp_kernels.h"
+#include "ndsp_mlp.h"
 
 static thread_local std::string g_err;
 
@@ -1151,6 +1152,11 @@ __global__ void k_conserved_final(const double* part, int nb, double* out, int n
 // reorder of the particle state into cell order
 // ===========================================================================
 
+__global__ void k_fill_f(float* __restrict__ a, int n, float v) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) a[k] = v;
+}
+
 __global__ void k_gather_d(const double* __restrict__ in, double* __restrict__ out, const int* sidx, int n) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) out[k] = in[sidx[k]];
@@ -1204,6 +1210,36 @@ __global__ void k_nb(KArgs a, Phys p, Grid g, int sym, const int* offs, int* out
     cnt++;
   });
   if (!offs) out[k] = cnt;
+}
+
+// ===========================================================================
+// h-predictor features and targets (MLP seed of the h-rho iteration)
+// ===========================================================================
+
+// after the predictor: features of each particle and the predictor's h
+__global__ void k_mlp_features(KArgs a, Phys p, double dt, double nexp, float* __restrict__ X) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  double** f = a.f;
+  const double hin = f[F_HIN][i], rho = f[F_RHO][i], m = f[F_M][i];
+  float* x = X + (size_t)i * MLP_LDX;
+  x[0] = (float)(100.0 * dt * f[F_DHDTIN][i] / hin);
+  x[1] = rho > 0.0 ? (float)(100.0 * dt * f[F_DRHODT][i] / rho) : 0.0f;
+  x[2] = (float)(1000.0 * f[F_MLPCORR][i]);
+  x[3] = (float)((a.numneigh[i] - nexp) / nexp);
+  x[4] = (float)(a.nits[i] - 1.5);
+  x[5] = rho > 0.0 ? (float)(1000.0 * (hin * pow(rho / m, p.dndim) / p.hfact - 1.0)) : 0.0f;
+  x[MLP_COL_HPRED] = (float)f[F_H][i];
+}
+
+// after the h-rho iteration: target log(h_converged / h_predictor)
+__global__ void k_mlp_target(KArgs a, float* __restrict__ X) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  float* x = X + (size_t)i * MLP_LDX;
+  const double corr = log(a.f[F_H][i] / (double)x[MLP_COL_HPRED]);
+  x[MLP_COL_TARGET] = (float)(MLP_TSCALE * corr);
+  a.f[F_MLPCORR][i] = corr;
 }
 
 // ===========================================================================
@@ -1265,6 +1301,14 @@ static int make_tables(int ikernel, int ndim, std::vector<double>& tab, double& 
   return 0;
 }
 
+// expected neighbour count of the smoothing length rule h = hfact (m/rho)^(1/d)
+static double nexp_of(const Phys& p) {
+  const double r = p.radkern * p.hfact;
+  if (p.ndim == 1) return 2.0 * r;
+  if (p.ndim == 2) return M_PI * r * r;
+  return 4.0 / 3.0 * M_PI * r * r * r;
+}
+
 struct ndsp_ctx {
   Phys p;
   int prec;
@@ -1289,6 +1333,8 @@ struct ndsp_ctx {
   int nsteps = 0;
   int reorder_every = 10;
   double dt = 0.0, time = 0.0;
+  MlpState mlp;
+  bool mlp_alloced = false;
   // timing
   bool timing = false;
   std::vector<std::string> tnames;
@@ -1719,6 +1765,7 @@ int ndsp_destroy(ndsp_ctx* c) {
                   a.D4, a.De, a.G0, a.G1, a.G2, a.G3, a.G4, a.G5, a.sc, (void*)a.tabf, (void*)a.tabd,
                   c->part, c->cons_out, a.cstart, a.cfill, c->bsum, c->list_a, c->list_b};
   for (void* q : ptrs) if (q) hipFree(q);
+  if (c->mlp_alloced) mlp_free(c->mlp);
   for (auto e : c->ev0) hipEventDestroy(e);
   for (auto e : c->ev1) hipEventDestroy(e);
   if (c->st) hipStreamDestroy(c->st);
@@ -1883,13 +1930,33 @@ int ndsp_step(ndsp_ctx* c, ndsp_stats* stats, ndsp_dt* dtout) {
     TScope t(c, "predict");
     hipLaunchKernelGGL(k_predict, dim3(nblk(c->n)), dim3(BS), 0, c->st, c->a, c->p, dt);
   }
+  MlpState& ml = c->mlp;
+  if (ml.enabled) {
+    TScope t(c, "mlp_seed");
+    hipLaunchKernelGGL(k_mlp_features, dim3(nblk(c->n)), dim3(BS), 0, c->st, c->a, c->p, dt, nexp_of(c->p), ml.X);
+    if (ml.have_params && ml.steps_seen >= ml.seed_after) mlp_launch_seed(ml, c->n, c->a.f[F_H], c->st);
+  }
   if (int e = do_linklist(c, 1)) return e;
   if (int e = do_density(c)) return e;
+  if (ml.enabled) hipLaunchKernelGGL(k_mlp_target, dim3(nblk(c->n)), dim3(BS), 0, c->st, c->a, ml.X);
   if (int e = do_rates(c)) return e;
   {
     TScope t(c, "correct");
     hipLaunchKernelGGL(k_correct, dim3(nblk(c->n)), dim3(BS), 0, c->st, c->a, c->p, dt);
     hipLaunchKernelGGL(k_boundary, dim3(nblk(c->n)), dim3(BS), 0, c->st, c->a, c->p, c->g, 1);
+  }
+  if (ml.enabled && ml.train) {
+    TScope t(c, "mlp_train");
+    // online training on this step's (features, converged h): Adam steps on
+    // pseudo-random subsets (stride coprime with n)
+    const int B = std::min(ml.batch, c->n);
+    for (int it = 0; it < ml.iters; it++) {
+      long long stride = 1000003;
+      while (std::__gcd(stride, (long long)c->n) != 1) stride += 2;
+      const long long off = (long long)(((unsigned long long)(c->nsteps + 1) * 2654435761ULL + it * 40503ULL) % c->n);
+      mlp_launch_grad(ml, ml.X, c->n, B, off, stride, c->st);
+      mlp_launch_adam(ml, ml.lr, c->st);
+    }
   }
   HIPCHK(hipGetLastError());
   DScal hs;
@@ -1898,6 +1965,7 @@ int ndsp_step(ndsp_ctx* c, ndsp_stats* stats, ndsp_dt* dtout) {
   c->time += dt;
   c->dt = hs.dt;
   c->nsteps++;
+  if (ml.enabled) ml.steps_seen++;
   fill_outputs(c, hs, stats, dtout);
   return check_err(c, hs);
 }
@@ -1977,4 +2045,129 @@ int ndsp_kernel_times(ndsp_ctx* c, double* ms, int32_t cap, int32_t* nout, char*
   return 0;
 }
 
+// ---- h-predictor MLP -------------------------------------------------------
+
+static int mlp_ready(ndsp_ctx* c) {
+  if (c->mlp_alloced) return 0;
+  HIPCHK(mlp_alloc(c->mlp, c->n));
+  c->mlp_alloced = true;
+  return 0;
+}
+
+int ndsp_mlp_config(ndsp_ctx* c, int32_t enable, int32_t train, double lr, int32_t batch, int32_t iters,
+                    int32_t seed_after) {
+  if (!c || batch < 1 || iters < 0 || seed_after < 0) return set_err(NDSP_ERR_BAD_ARG, "mlp_config: bad arguments");
+  if (int e = mlp_ready(c)) return e;
+  MlpState& m = c->mlp;
+  m.enabled = enable != 0;
+  m.train = train != 0;
+  m.lr = (float)lr;
+  m.batch = batch;
+  m.iters = iters;
+  m.seed_after = seed_after;
+  if (m.enabled && !m.have_params) {
+    // Glorot-uniform weights, zero biases (same layout as ndsp_mlp_set_params)
+    std::vector<float> w(MLP_NPARAM, 0.0f);
+    unsigned long long st = 0x9E3779B97F4A7C15ULL;
+    auto uni = [&]() {
+      st = st * 6364136223846793005ULL + 1442695040888963407ULL;
+      return (double)(st >> 11) * (1.0 / 9007199254740992.0);
+    };
+    int o = 0;
+    const int shapes[3][2] = {{MLP_NH, MLP_NIN}, {MLP_NH, MLP_NH}, {1, MLP_NH}};
+    for (auto& sh : shapes) {
+      const double lim = std::sqrt(6.0 / (sh[0] + sh[1]));
+      for (int k = 0; k < sh[0] * sh[1]; k++) w[o + k] = (float)((2.0 * uni() - 1.0) * lim);
+      o += sh[0] * sh[1] + sh[0];
+    }
+    HIPCHK(mlp_set_params(m, w.data(), c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+  }
+  return 0;
+}
+
+int ndsp_mlp_set_params(ndsp_ctx* c, const float* params, int32_t n) {
+  if (!c || !params || n != MLP_NPARAM) return set_err(NDSP_ERR_BAD_ARG, "mlp_set_params: need 1313 floats");
+  if (int e = mlp_ready(c)) return e;
+  HIPCHK(mlp_set_params(c->mlp, params, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int ndsp_mlp_get_params(ndsp_ctx* c, float* params, int32_t n, int32_t* adam_steps) {
+  if (!c || !params || n != MLP_NPARAM) return set_err(NDSP_ERR_BAD_ARG, "mlp_get_params: need 1313 floats");
+  if (int e = mlp_ready(c)) return e;
+  HIPCHK(hipMemcpyAsync(params, c->mlp.w32, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (adam_steps) *adam_steps = c->mlp.adam_steps;
+  return 0;
+}
+
+static int mlp_stage(ndsp_ctx* c, const float* X, const float* T, int32_t B) {
+  MlpState& m = c->mlp;
+  if (B > m.xext_rows) {
+    if (m.Xext) (void)hipFree(m.Xext);
+    m.Xext = nullptr;
+    HIPCHK(hipMalloc(&m.Xext, sizeof(float) * MLP_LDX * (size_t)B));
+    m.xext_rows = B;
+  }
+  std::vector<float> buf((size_t)B * MLP_LDX, 0.0f);
+  for (int b = 0; b < B; b++) {
+    for (int q = 0; q < MLP_NIN; q++) buf[(size_t)b * MLP_LDX + q] = X[(size_t)b * MLP_NIN + q];
+    if (T) buf[(size_t)b * MLP_LDX + MLP_COL_TARGET] = T[b];
+  }
+  HIPCHK(hipMemcpy(m.Xext, buf.data(), sizeof(float) * buf.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int ndsp_mlp_forward(ndsp_ctx* c, const float* X, int32_t B, float* y) {
+  if (!c || !X || !y || B < 1) return set_err(NDSP_ERR_BAD_ARG, "mlp_forward: bad arguments");
+  if (int e = mlp_ready(c)) return e;
+  if (int e = mlp_stage(c, X, nullptr, B)) return e;
+  float* dy = nullptr;
+  HIPCHK(hipMalloc(&dy, sizeof(float) * B));
+  mlp_launch_forward(c->mlp, c->mlp.Xext, MLP_LDX, B, dy, c->st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(y, dy, sizeof(float) * B, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  (void)hipFree(dy);
+  return 0;
+}
+
+int ndsp_mlp_grad(ndsp_ctx* c, const float* X, const float* T, int32_t B, float* grad, double* loss) {
+  if (!c || !X || !T || !grad || B < 1) return set_err(NDSP_ERR_BAD_ARG, "mlp_grad: bad arguments");
+  if (int e = mlp_ready(c)) return e;
+  if (int e = mlp_stage(c, X, T, B)) return e;
+  MlpState& m = c->mlp;
+  hipLaunchKernelGGL(k_fill_f, dim3(nblk(MLP_NPARAM)), dim3(BS), 0, c->st, m.grad, (int)MLP_NPARAM, 0.0f);
+  mlp_launch_grad(m, m.Xext, B, B, 0, 1, c->st);
+  HIPCHK(hipGetLastError());
+  float l = 0.f;
+  HIPCHK(hipMemcpyAsync(grad, m.grad, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(&l, m.loss, sizeof(float), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  hipLaunchKernelGGL(k_fill_f, dim3(nblk(MLP_NPARAM)), dim3(BS), 0, c->st, m.grad, (int)MLP_NPARAM, 0.0f);
+  if (loss) *loss = l;
+  return 0;
+}
+
+int ndsp_mlp_train(ndsp_ctx* c, const float* X, const float* T, int32_t B, int32_t iters, double lr,
+                   double* loss) {
+  if (!c || !X || !T || B < 1 || iters < 1) return set_err(NDSP_ERR_BAD_ARG, "mlp_train: bad arguments");
+  if (int e = mlp_ready(c)) return e;
+  if (int e = mlp_stage(c, X, T, B)) return e;
+  MlpState& m = c->mlp;
+  float l = 0.f;
+  for (int it = 0; it < iters; it++) {
+    mlp_launch_grad(m, m.Xext, B, B, 0, 1, c->st);
+    HIPCHK(hipMemcpyAsync(&l, m.loss, sizeof(float), hipMemcpyDeviceToHost, c->st));
+    mlp_launch_adam(m, (float)lr, c->st);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (loss) *loss = l;
+  return 0;
+}
+
 }  // extern "C"
+
