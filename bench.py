@@ -41,10 +41,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+_SETUPS = {}
+
+
+def setup(nside):
+    from ndspmhd_amd.setups import periodic_mhd_box
+    if nside not in _SETUPS:
+        _SETUPS.clear()
+        _SETUPS[nside] = periodic_mhd_box(nside)
+    return _SETUPS[nside]
+
+
 def run_gpu(nside, steps, warmup, prec, timing=True):
     import ndspmhd_amd as nd
-    from ndspmhd_amd.setups import periodic_mhd_box
-    o, st = periodic_mhd_box(nside)
+    o, st = setup(nside)
     sim = nd.Simulation(o, st, precision=prec)
     sim.derivs()
     for _ in range(warmup):
@@ -123,7 +133,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nside", type=int, default=128)
     ap.add_argument("--fp64-steps", type=int, default=5)
-    ap.add_argument("--cpu-nside", type=int, default=40)
+    ap.add_argument("--cpu-nside", type=int, default=96)
+    ap.add_argument("--big-nside", type=int, default=256,
+                    help="second workload (mixed vs plain FP64 steps/s); 0 skips it")
+    ap.add_argument("--big-steps", type=int, default=5)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after the timed region, one device copy of this many MiB "
+                         "(reference kernel for scaling rocprof FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     args = ap.parse_args()
@@ -187,6 +203,29 @@ def main():
                               "speedup_mixed_over_fp64": value / v64}
         r64["sim"].close()
         result["error_vs_fp64"] = field_errors(min(args.nside, 64))
+    if args.big_nside > 0 and not args.no_fp64:
+        nb = args.big_nside
+        log("[bench] %d^3 mixed vs plain FP64, %d steps each" % (nb, args.big_steps))
+        big = {}
+        for prec, key in ((0, "mixed"), (1, "fp64")):
+            rb = run_gpu(nb, args.big_steps, 1, prec=prec, timing=False)
+            big[key] = {"steps_per_s": args.big_steps / rb["wall"],
+                        "pairs_per_s": (rb["pairs_density"] + rb["pairs_force"]) / rb["wall"],
+                        "ncalc_mean": rb["ncalc_mean"]}
+            rb["sim"].close()
+            del rb
+        big["speedup_mixed_over_fp64"] = big["mixed"]["steps_per_s"] / big["fp64"]["steps_per_s"]
+        big["particles"] = nb ** 3
+        result["box_%d" % nb] = big
+    if args.copy_probe_mib > 0:
+        import torch
+        nbytes = args.copy_probe_mib << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        result["copy_probe_bytes"] = nbytes
+        del src, dst
     if not args.no_cpu:
         log("[bench] CPU baseline on %d^3" % args.cpu_nside)
         result["cpu_baseline"] = cpu_baseline(args.cpu_nside)

@@ -14,7 +14,9 @@
  * Option vectors (same slots as the reference harness and the oracle):
  *   iopt[32]: 0 ikernel, 1 iener, 2 imhd, 3 imagforce, 4 idivbzero, 5 iav,
  *             6-8 iavlim(1:3), 9 idust, 10 idrag_nature, 11-13 ibound(1:3),
- *             14 maxdensits, 15 idustevol, 16 ndim
+ *             14 maxdensits, 15 idustevol, 16 ndim,
+ *             17 cellscan (device-only: 1 = scan the cell grid in every
+ *                pass instead of per-particle neighbour lists)
  *   dopt[32]: 0 hfact, 1 tolh, 2 gamma, 3 polyk, 4 Kdrag, 5 alphamin,
  *             6 alphaumin, 7 alphaBmin, 8 beta, 9 avdecayconst,
  *             10 psidecayfact, 11 C_cour, 12 C_force, 13-15 xmin(1:3),

@@ -30,8 +30,12 @@
 #include <climits>
 #include <cmath>
 #include <cstdio>
+#include <cstddef>
+#include <cstdlib>
+#include <initializer_list>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "ndsp.h"
@@ -119,18 +123,34 @@ __global__ void k_binstats(KArgs a, int ndim, double* part) {
   if (t < 7) part[blockIdx.x * 8 + t] = sm[t][0];
 }
 
-__global__ void k_binstats_final(DScal* sc, const double* part, int nb) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double hm = 0.0, lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
-  for (int b = 0; b < nb; b++) {
-    hm = fmax(hm, part[b * 8]);
+// one block of BS threads folds the per-block partials of k_binstats
+__global__ __launch_bounds__(BS) void k_binstats_final(DScal* sc, const double* part, int nb) {
+  __shared__ double sm[7][BS];
+  double v[7] = {0.0, DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    v[0] = fmax(v[0], part[b * 8]);
     for (int d = 0; d < 3; d++) {
-      lo[d] = fmin(lo[d], part[b * 8 + 1 + d]);
-      hi[d] = fmax(hi[d], part[b * 8 + 4 + d]);
+      v[1 + d] = fmin(v[1 + d], part[b * 8 + 1 + d]);
+      v[4 + d] = fmax(v[4 + d], part[b * 8 + 4 + d]);
     }
   }
-  sc->hmax_bin = hm;
-  for (int d = 0; d < 3; d++) { sc->bbox_lo[d] = lo[d]; sc->bbox_hi[d] = hi[d]; }
+  for (int q = 0; q < 7; q++) sm[q][threadIdx.x] = v[q];
+  __syncthreads();
+  for (int st = BS / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      const int t = threadIdx.x;
+      sm[0][t] = fmax(sm[0][t], sm[0][t + st]);
+      for (int d = 0; d < 3; d++) {
+        sm[1 + d][t] = fmin(sm[1 + d][t], sm[1 + d][t + st]);
+        sm[4 + d][t] = fmax(sm[4 + d][t], sm[4 + d][t + st]);
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    sc->hmax_bin = sm[0][0];
+    for (int d = 0; d < 3; d++) { sc->bbox_lo[d] = sm[1 + d][0]; sc->bbox_hi[d] = sm[4 + d][0]; }
+  }
 }
 
 __device__ __forceinline__ int cell_coord(const Grid& g, int d, double x) {
@@ -274,57 +294,389 @@ __global__ void k_pack_force(KArgs a, Phys p) {
 // neighbour iteration over image cells
 // ===========================================================================
 
-// Calls body(kk, xj_img[3], flip[3], is_image) for every particle of every
-// cell image overlapping [xi - rad, xi + rad]; xj_img holds the image
-// coordinates (ghost arithmetic of src/ghostND_mhd.f90:180-200).
+// Calls body(kk, xj_img[3], flip[3], code) for every particle of every cell
+// image that can hold a point within rad of xi; xj_img holds the image
+// coordinates (ghost arithmetic of src/ghostND_mhd.f90:180-200), flip[d] = 1
+// where a reflecting image reverses the velocity component, code = the image
+// kinds kx + 5 ky + 25 kz (0 for a particle seen in place).
 template <int NDIM, typename F>
 __device__ __forceinline__ void for_each_candidate(const KArgs& a, const Grid& g, const double* xi,
                                                    double rad, F&& body) {
+  // rows (cy, cz) whose cells lie wholly outside the sphere are skipped and
+  // each row is cut to the chord of the sphere; cell faces are widened by a
+  // relative margin so that a particle binned by rounding into a cell is
+  // never culled (the FP64 pair test downstream is what decides membership)
+  const double rad2 = rad * rad * (1.0 + 1.0e-12);
   int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-  for (int d = 0; d < NDIM; d++) cell_range(g, d, xi[d], rad, lo[d], hi[d]);
+  for (int d = 1; d < NDIM; d++) cell_range(g, d, xi[d], rad, lo[d], hi[d]);
   for (int cz = lo[2]; cz <= hi[2]; cz++) {
     int kz = 0, sz = 0;
-    if (NDIM >= 3) { sz = cell_image(g, 2, cz, kz); if (sz < 0) continue; }
+    double gz2 = 0.0;
+    if (NDIM >= 3) {
+      sz = cell_image(g, 2, cz, kz);
+      if (sz < 0) continue;
+      const double gz = cell_gap(g, 2, cz, xi[2]);
+      gz2 = gz * gz;
+      if (gz2 > rad2) continue;
+    }
     for (int cy = lo[1]; cy <= hi[1]; cy++) {
       int ky = 0, sy = 0;
-      if (NDIM >= 2) { sy = cell_image(g, 1, cy, ky); if (sy < 0) continue; }
+      double r2 = rad2 - gz2;
+      if (NDIM >= 2) {
+        sy = cell_image(g, 1, cy, ky);
+        if (sy < 0) continue;
+        const double gy = cell_gap(g, 1, cy, xi[1]);
+        r2 -= gy * gy;
+        if (r2 < 0.0) continue;
+      }
+      int lx, hx;
+      cell_range(g, 0, xi[0], NDIM >= 2 ? sqrt(r2) : rad, lx, hx);
       const int rowbase = g.nc[0] * (sy + g.nc[1] * sz);
-      if (lo[0] >= 0 && hi[0] < g.nc[0] && kz == 0 && ky == 0) {
+      const int flip[3] = {0, ky >= 3, kz >= 3};
+      if (lx >= 0 && hx < g.nc[0] && kz == 0 && ky == 0) {
         // contiguous run of cells with no image: one particle range
-        const int b = a.cstart[rowbase + lo[0]], e = a.cstart[rowbase + hi[0] + 1];
+        const int b = a.cstart[rowbase + lx], e = a.cstart[rowbase + hx + 1];
         for (int kk = b; kk < e; kk++) {
-          double4 pj = a.P4[kk];
-          double xj[3] = {pj.x, pj.y, pj.z};
-          const int flip[3] = {0, 0, 0};
-          body(kk, xj, flip, false);
+          const double4 pj = a.P4[kk];
+          const double xj[3] = {pj.x, pj.y, pj.z};
+          body(kk, xj, flip, 0);
         }
         continue;
       }
-      for (int cx = lo[0]; cx <= hi[0]; cx++) {
+      for (int cx = lx; cx <= hx; cx++) {
         int kx = 0;
-        int sx = cell_image(g, 0, cx, kx);
+        const int sx = cell_image(g, 0, cx, kx);
         if (sx < 0) continue;
         const int kind[3] = {kx, ky, kz};
-        const bool img = (kx | ky | kz) != 0;
+        const int code = kx + 5 * ky + 25 * kz;
+        const int fl[3] = {kx >= 3, ky >= 3, kz >= 3};
         const int b = a.cstart[rowbase + sx], e = a.cstart[rowbase + sx + 1];
         for (int kk = b; kk < e; kk++) {
-          double4 pj = a.P4[kk];
+          const double4 pj = a.P4[kk];
           double xj[3] = {pj.x, pj.y, pj.z};
-          int flip[3] = {0, 0, 0};
           bool ok = true;
-          if (img) {
-            for (int d = 0; d < NDIM; d++) {
-              if (!kind[d]) continue;
-              ok = ok && image_coord(g, d, kind[d], xj[d], a.dxb[kk], xj[d]);
-              flip[d] = kind[d] >= 3;
-            }
+          if (code) {
+            for (int d = 0; d < NDIM; d++)
+              if (kind[d]) ok = ok && image_coord(g, d, kind[d], xj[d], a.dxb[kk], xj[d]);
           }
-          if (ok) body(kk, xj, flip, img);
+          if (ok) body(kk, xj, fl, code);
         }
       }
     }
   }
 }
+
+// Same contract as for_each_candidate, over the first cnt entries of the
+// neighbour list of sorted particle k; image coordinates are recomputed with
+// the ghost arithmetic, so the exact FP64 tests downstream see the same values.
+template <int NDIM, typename F>
+__device__ __forceinline__ void for_each_listed(const KArgs& a, const Grid& g, int k, int cnt, F&& body) {
+  // particle-major lists, four entries per load
+  const uint4* L = (const uint4*)(a.nbl + (size_t)k * a.maxnb);
+  for (int m0 = 0; m0 < cnt; m0 += 4) {
+    const uint4 e4 = L[m0 >> 2];
+    const unsigned ev[4] = {e4.x, e4.y, e4.z, e4.w};
+    const int mm = min(4, cnt - m0);
+    for (int u = 0; u < mm; u++) {
+      const unsigned e = ev[u];
+      const int kk = (int)(e >> 7);
+      const int code = (int)(e & 127u);
+      const double4 pj = a.P4[kk];
+      double xj[3] = {pj.x, pj.y, pj.z};
+      int flip[3] = {0, 0, 0};
+      bool ok = true;
+      if (code) {
+        const int kind[3] = {code % 5, (code / 5) % 5, code / 25};
+        for (int d = 0; d < NDIM; d++) {
+          if (!kind[d]) continue;
+          ok = ok && image_coord(g, d, kind[d], xj[d], a.dxb[kk], xj[d]);
+          flip[d] = kind[d] >= 3;
+        }
+      }
+      if (ok) body(kk, xj, flip, code);
+    }
+  }
+}
+
+// Neighbour lists (periodic or open domains).  Entry m of sorted particle k
+// sits at nbl[k*maxnb + m] = (sorted index << 7) | image code.  Entries
+// [0, nbc) are the gather list G(k): every j with r_kj < radkern*h_k*(1+skin),
+// in the candidate order of the cell scan; then nbx transposed extras: every
+// j with k in G(j) but j not in G(k).  Together they hold every pair with
+// r < radkern*max(h_k, h_j) while no h has grown past hcap = h*(1+skin).
+// Membership uses the minimum-image separation of the raw positions, which is
+// exactly antisymmetric in FP64 (fl(a-b) = -fl(b-a), and so are the +-L
+// shifts), so "k in G(j)" is decided identically from k's and from j's side
+// and no pair is lost or doubled.  Lists need 2*radkern*hmax*(1+skin) < L along
+// periodic dimensions (checked by the host before every build), so the only
+// copy of j within the list radius is its minimum image.
+template <int NDIM>
+__device__ __forceinline__ double min_image_r2(const Grid& g, const double* xa, const double* xb) {
+#pragma clang fp contract(off)
+  double r2 = 0.0;
+  for (int d = 0; d < NDIM; d++) {
+    double dx = xa[d] - xb[d];
+    if (g.bound[d] == 3) {
+      const double L = g.xmax[d] - g.xmin[d];
+      if (dx > 0.5 * L) dx = dx - L;
+      else if (dx < -0.5 * L) dx = dx + L;
+    }
+    r2 = r2 + dx * dx;
+  }
+  return r2;
+}
+
+__device__ __forceinline__ int wave_min_i(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
+// image code of k seen from j, given the code of j seen from k (periodic
+// kinds 1 <-> 2 per dimension)
+__device__ __forceinline__ unsigned inverse_code(unsigned code) {
+  unsigned r = 0, m = 1;
+  for (int d = 0; d < 3; d++) {
+    const unsigned kd = code % 5;
+    code /= 5;
+    r += m * (kd == 1 ? 2u : kd == 2 ? 1u : kd);
+    m *= 5;
+  }
+  return r;
+}
+
+// One wavefront per 64 consecutive sorted particles.  The wave walks the
+// union of its particles' gather boxes row by row (rows (cy, cz) in the
+// nested order of for_each_candidate, so each list keeps the scan's order);
+// a row segment of one image kind is a contiguous particle range, staged 64
+// candidates at a time in LDS.  Each lane first marks the candidates within
+// the list radius by an FP32 test on FP64 separations (relative margin 1e-4),
+// then applies the exact FP64 minimum-image test to the marked ones.  A pair
+// k-j with j in G(k) and k not in G(j) is appended to the extras of j, which
+// fill j's row from its end (k_nblist_sort moves them into place).
+template <int NDIM>
+__global__ __launch_bounds__(64) void k_nblist(KArgs a, Phys p, Grid g) {
+  __shared__ double4 sx[64];
+  __shared__ unsigned se[64];
+  const int lane = threadIdx.x;
+  const int k = blockIdx.x * 64 + lane;
+  const bool act = k < a.n;
+  const double s1 = 1.0 + p.skin;
+  const double lim2 = p.radkern2 * s1 * s1 * (1.0 + 1.0e-12);
+  const float lim2f = (float)(p.radkern2 * s1 * s1 * (1.0 + 1.0e-4));
+  const double4 pk = a.P4[act ? k : 0];
+  const double xi[3] = {pk.x, pk.y, pk.z};
+  const double hi21 = pk.w * pk.w;
+  const float hi2f = (float)hi21;
+  const double rad = act ? p.radkern * s1 * (1.0 + 1.0e-9) / pk.w : 0.0;
+  const double rad2 = rad * rad * (1.0 + 1.0e-12);
+  unsigned* L = a.nbl + (size_t)(act ? k : 0) * a.maxnb;
+  int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  for (int d = 1; d < NDIM; d++) {
+    int l = INT_MAX, h = INT_MIN;
+    if (act) cell_range(g, d, xi[d], rad, l, h);
+    lo[d] = wave_min_i(l);
+    hi[d] = wave_max_i(h);
+  }
+  int cnt = 0;
+  bool over = false;
+  for (int cz = lo[2]; cz <= hi[2]; cz++) {
+    int kz = 0, sz = 0;
+    double gz2 = 0.0;
+    if (NDIM >= 3) {
+      sz = cell_image(g, 2, cz, kz);
+      if (sz < 0) continue;
+      const double gz = cell_gap(g, 2, cz, xi[2]);
+      gz2 = gz * gz;
+    }
+    for (int cy = lo[1]; cy <= hi[1]; cy++) {
+      int ky = 0, sy = 0;
+      double r2 = rad2 - gz2;
+      if (NDIM >= 2) {
+        sy = cell_image(g, 1, cy, ky);
+        if (sy < 0) continue;
+        const double gy = cell_gap(g, 1, cy, xi[1]);
+        r2 -= gy * gy;
+      }
+      int lx = INT_MAX, hx = INT_MIN;
+      if (act && r2 >= 0.0) cell_range(g, 0, xi[0], NDIM >= 2 ? sqrt(r2) : rad, lx, hx);
+      lx = wave_min_i(lx);
+      hx = wave_max_i(hx);
+      if (lx > hx) continue;
+      const int rowbase = g.nc[0] * (sy + g.nc[1] * sz);
+      const int nx = g.nc[0];
+      for (int sgi = 0; sgi < 3; sgi++) {
+        const int c0 = sgi == 0 ? lx : sgi == 1 ? max(lx, 0) : max(lx, nx);
+        const int c1 = sgi == 0 ? min(hx, -1) : sgi == 1 ? min(hx, nx - 1) : hx;
+        if (c0 > c1) continue;
+        int kx = 0, kx1 = 0;
+        const int s0 = cell_image(g, 0, c0, kx);
+        const int s1c = cell_image(g, 0, c1, kx1);
+        if (s0 < 0 || s1c < 0) continue;
+        const int kind[3] = {kx, ky, kz};
+        const unsigned code = (unsigned)(kx + 5 * ky + 25 * kz);
+        // this lane's position shifted by minus the image offset of the segment
+        double xs[3] = {xi[0], xi[1], xi[2]};
+        for (int d = 0; d < NDIM; d++) {
+          const double Ld = g.xmax[d] - g.xmin[d];
+          if (kind[d] == 1) xs[d] = xi[d] + Ld;
+          else if (kind[d] == 2) xs[d] = xi[d] - Ld;
+        }
+        const int pb = a.cstart[rowbase + s0], pe = a.cstart[rowbase + s1c + 1];
+        for (int base = pb; base < pe; base += 64) {
+          const int m = min(64, pe - base);
+          if (lane < m) {
+            sx[lane] = a.P4[base + lane];
+            se[lane] = ((unsigned)(base + lane) << 7) | code;
+          }
+          __syncthreads();
+          if (act) {
+            unsigned long long mask = 0ull;
+            for (int t = 0; t < m; t++) {
+              const double4 c = sx[t];
+              const float dx = (float)(xs[0] - c.x);
+              const float dy = NDIM >= 2 ? (float)(xs[1] - c.y) : 0.f;
+              const float dz = NDIM >= 3 ? (float)(xs[2] - c.z) : 0.f;
+              const float r2f = dx * dx + dy * dy + dz * dz;
+              if (r2f * hi2f < lim2f) mask |= 1ull << t;
+            }
+            while (mask) {
+              const int t = __builtin_ctzll(mask);
+              mask &= mask - 1ull;
+              const double4 c = sx[t];
+              const double xr[3] = {c.x, c.y, c.z};
+              const double rr = min_image_r2<NDIM>(g, xi, xr);
+              if (!(rr * hi21 < lim2)) continue;
+              const unsigned e = se[t];
+              if (cnt < a.maxnb) L[cnt] = e;
+              else over = true;
+              cnt++;
+              const int kk = (int)(e >> 7);
+              // same operands as kk's own test of k (rr is symmetric)
+              if (kk != k && !(rr * (c.w * c.w) < lim2)) {
+                const int slot = atomicAdd(&a.nbx[kk], 1);
+                if (slot < a.maxnb)
+                  a.nbl[(size_t)kk * a.maxnb + (a.maxnb - 1 - slot)] = ((unsigned)k << 7) | inverse_code(code);
+              }
+            }
+          }
+          __syncthreads();
+        }
+      }
+    }
+  }
+  if (act) {
+    a.nbc[k] = over ? -1 : cnt;
+    a.hcap[k] = s1 / pk.w;
+  }
+}
+
+// extras (written from the end of the row) into [nbc, nbc + nbx) in
+// ascending entry order, so sums are deterministic; a row that did not hold
+// the gather list and the extras falls back to the cell scan
+__global__ __launch_bounds__(BS) void k_nblist_sort(KArgs a) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= a.n) return;
+  const int c0 = a.nbc[k], nx = a.nbx[k];
+  if (c0 < 0 || c0 + nx > a.maxnb) {
+    if (c0 >= 0) a.nbc[k] = -1;
+    atomicAdd(&a.sc->list_overflow, 1);
+    return;
+  }
+  // extras occupy E = L[maxnb-nx .. maxnb-1]: insertion sort in place, then
+  // move down to L[c0 ..] (ascending copy, destination not above source)
+  unsigned* E = a.nbl + (size_t)k * a.maxnb + (a.maxnb - nx);
+  for (int m = 1; m < nx; m++) {
+    const unsigned v = E[m];
+    int q = m - 1;
+    while (q >= 0 && E[q] > v) {
+      E[q + 1] = E[q];
+      q--;
+    }
+    E[q + 1] = v;
+  }
+  unsigned* D = a.nbl + (size_t)k * a.maxnb + c0;
+  if (D != E)
+    for (int m = 0; m < nx; m++) D[m] = E[m];
+}
+
+// ===========================================================================
+// global reductions without contended atomics: every block reduces its
+// values (wave shuffles, then LDS across waves) and writes one partial per
+// quantity; k_red_final folds the partials into the scalar record.
+// ===========================================================================
+
+__device__ __forceinline__ double red_combine(int kind, double a, double b) {
+  return kind == RED_SUM ? a + b : kind == RED_MAX ? fmax(a, b) : fmin(a, b);
+}
+
+// v[q] of every thread of the block -> red[q * rcap + blockIdx.x]
+// (kinds are compile-time per call site; every thread must call this)
+template <int NQ>
+__device__ __forceinline__ void block_partials(double (&v)[NQ], const int (&kind)[NQ], double* red,
+                                               int rcap) {
+  __shared__ double sred[NQ][BS / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    double x = v[q];
+    for (int o = 32; o > 0; o >>= 1) x = red_combine(kind[q], x, __shfl_xor(x, o));
+    if (lane == 0) sred[q][wv] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < NQ) {
+    const int q = threadIdx.x;
+    double x = sred[q][0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) x = red_combine(kind[q], x, sred[q][w]);
+    red[(size_t)q * rcap + blockIdx.x] = x;
+  }
+}
+
+__global__ __launch_bounds__(BS) void k_red_final(const double* __restrict__ red, int rcap, int nblocks,
+                                                  RedSpec spec, DScal* sc) {
+  __shared__ double sm[BS];
+  for (int q = 0; q < spec.nq; q++) {
+    const int kind = spec.op[q] & 15, type = spec.op[q] >> 4;
+    const double neutral = kind == RED_SUM ? 0.0 : kind == RED_MAX ? -DBL_MAX : DBL_MAX;
+    double x = neutral;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) x = red_combine(kind, x, red[(size_t)q * rcap + b]);
+    sm[threadIdx.x] = x;
+    __syncthreads();
+    for (int s = BS / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) sm[threadIdx.x] = red_combine(kind, sm[threadIdx.x], sm[threadIdx.x + s]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      char* base = (char*)sc + spec.off[q];
+      const double r = sm[0];
+      if (type == RED_F64) {
+        double* d = (double*)base;
+        *d = red_combine(kind, *d, r);
+      } else if (type == RED_I32) {
+        int* d = (int*)base;
+        *d = (int)red_combine(kind, (double)*d, r);
+      } else {
+        unsigned long long* d = (unsigned long long*)base;
+        *d = kind == RED_SUM ? *d + (unsigned long long)r : (unsigned long long)red_combine(kind, (double)*d, r);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+static RedSpec red_spec(std::initializer_list<std::pair<int, int>> q) {
+  RedSpec s{};
+  for (auto& e : q) {
+    s.op[s.nq] = e.first;
+    s.off[s.nq] = e.second;
+    s.nq++;
+  }
+  return s;
+}
+#define RED_Q(kind, type, field) std::make_pair((kind) | ((type) << 4), (int)offsetof(DScal, field))
 
 // ===========================================================================
 // density: h-rho Newton-Raphson, src/iterate_density.f90:43-360
@@ -378,7 +730,7 @@ __device__ __forceinline__ void nr_update(const Phys& p, double hh, double hin, 
   r.dhdrho = dhdrhoi;
 }
 
-template <int NDIM, typename PT>
+template <int NDIM, typename PT, bool DUST>
 __global__ __launch_bounds__(BS) void k_nr_pass(KArgs a, Phys p, Grid g, const int* __restrict__ list, int nlist,
                                                 int its0, int sym, int persistent, int* __restrict__ next_list) {
   __shared__ PT tw[NDSP_IKERN + 1];
@@ -400,6 +752,8 @@ __global__ __launch_bounds__(BS) void k_nr_pass(KArgs a, Phys p, Grid g, const i
   typedef typename V4<PT>::type T4;
   const T4* D4 = (const T4*)a.D4;
   const PT* De = (const PT*)a.De;
+  // this thread's share of the global reductions
+  double r_stress = 0.0, r_hmax = 0.0, r_its = 0.0, r_nmin = DBL_MAX, r_nmax = 0.0, r_pairs = 0.0, r_ncalc = 0.0;
   if (active) {
     const int i = a.sidx[k];
     const double4 pk = a.P4[k];
@@ -414,7 +768,7 @@ __global__ __launch_bounds__(BS) void k_nr_pass(KArgs a, Phys p, Grid g, const i
     const PT ddq2 = (PT)p.ddq2table, dq2 = (PT)p.dq2table;
     const double R2 = p.radkern2;
     const double hmaxc = a.sc->hmax_cur;
-    const bool dust = p.idust != 0;
+    constexpr bool dust = DUST;
 
     int its = its0 - 1, nn = 0;
     unsigned long long npairs = 0;
@@ -437,7 +791,11 @@ __global__ __launch_bounds__(BS) void k_nr_pass(KArgs a, Phys p, Grid g, const i
       int nsym = 0, ngat = 0;
       const double rad = p.radkern * (symp ? fmax(h, hmaxc) : h);
       if (!(rad < p.boxmin)) { err = NDSP_ERR_CELLS; done = true; break; }
-      for_each_candidate<NDIM>(a, g, xi, rad, [&](int kk, const double* xj, const int* flip, bool img) {
+      // the list serves the gather while h stays within its margin, and the
+      // symmetric count of the first pass (neighbours still at their build h)
+      const bool uselist = p.use_list && a.nbc[k] >= 0 && h <= a.hcap[k] && (!symp || its == 1);
+      const int nlisted = uselist ? (symp ? a.nbc[k] + a.nbx[k] : a.nbc[k]) : 0;
+      auto body = [&](int kk, const double* xj, const int* flip, int code) {
         double dx[3] = {0.0, 0.0, 0.0};
         for (int d = 0; d < NDIM; d++) dx[d] = xi[d] - xj[d];
         const double rij2 = rij2_exact<NDIM>(dx);
@@ -467,14 +825,16 @@ __global__ __launch_bounds__(BS) void k_nr_pass(KArgs a, Phys p, Grid g, const i
           rhod.add(mj * epsj * wabi);
           rhog.add(mj * (PT(1) - epsj) * wabi);
         }
-        if (kk != k || img) {
+        if (kk != k || code != 0) {
           const PT rinv = PT(1) / (rij + (PT)DBL_EPSILON);
           const PT vj[3] = {flip[0] ? -dj.x : dj.x, flip[1] ? -dj.y : dj.y, flip[2] ? -dj.z : dj.z};
           PT dvdotr = PT(0);
           for (int d = 0; d < NDIM; d++) dvdotr += (vi[d] - vj[d]) * ((PT)dx[d] * rinv);
           drhodt.add(mj * dvdotr * grkerni);
         }
-      });
+      };
+      if (uselist) for_each_listed<NDIM>(a, g, k, nlisted, body);
+      else for_each_candidate<NDIM>(a, g, xi, rad, body);
       npairs += (unsigned long long)ngat;
       nn = symp ? nsym : ngat;
       // Newton-Raphson update, src/iterate_density.f90:162-278 (FP64)
@@ -548,16 +908,22 @@ __global__ __launch_bounds__(BS) void k_nr_pass(KArgs a, Phys p, Grid g, const i
       if (p.imhd != 0) {
         const double bx = a.f[F_BX][i], by = a.f[F_BY][i], bz = a.f[F_BZ][i];
         const double B2 = bx * bx + by * by + bz * bz;
-        atomic_max_pos(&a.sc->stressmax, fmax(0.5 * B2 - pr, 0.0));
+        r_stress = fmax(0.5 * B2 - pr, 0.0);
       }
-      atomic_max_pos(&a.sc->hmax_nr, h);
-      atomicMax(&a.sc->itsmax, its);
-      atomicMin(&a.sc->nmin, nn);
-      atomicMax(&a.sc->nmax, nn);
+      r_hmax = h;
+      if (p.use_list && h > a.hcap[k]) atomicOr(&a.sc->list_invalid, 1);
+      r_its = its;
+      r_nmin = nn;
+      r_nmax = nn;
     }
-    atomicAdd(&a.sc->pairs_density, npairs);
-    atomicAdd(&a.sc->ncalc_total, (unsigned long long)(its - its0 + 1));
+    r_pairs = (double)npairs;
+    r_ncalc = its - its0 + 1;
     if (err) atomicCAS(&a.sc->err, 0, err);
+  }
+  {
+    double rv[7] = {r_stress, r_hmax, r_its, r_nmin, r_nmax, r_pairs, r_ncalc};
+    const int rk[7] = {RED_MAX, RED_MAX, RED_MAX, RED_MIN, RED_MAX, RED_SUM, RED_SUM};
+    block_partials<7>(rv, rk, a.red, a.rcap);
   }
   // append unconverged particles to the next pass's list (one atomic per wave)
   if (!persistent) {
@@ -610,18 +976,8 @@ __global__ void k_next_pass_scalars(DScal* sc, int relink) {
 // force: fused pair sums of get_rates
 // ===========================================================================
 
-template <typename PT>
-__device__ __forceinline__ PT wave_max(PT v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-  return v;
-}
-template <typename PT>
-__device__ __forceinline__ PT wave_min(PT v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
-  return v;
-}
-
-template <int NDIM, typename PT>
+// PHYS: 0 hydro, 1 MHD (imhd 11), 2 one-fluid dust (idust 4)
+template <int NDIM, typename PT, int PHYS>
 __global__ __launch_bounds__(BS) void k_force(KArgs a, Phys p, Grid g) {
   __shared__ PT tg[NDSP_IKERN + 1];
   {
@@ -638,8 +994,8 @@ __global__ __launch_bounds__(BS) void k_force(KArgs a, Phys p, Grid g) {
   const T4* G3 = (const T4*)a.G3;
   const T4* G4 = (const T4*)a.G4;
   const PT* G5 = (const PT*)a.G5;
-  const bool mhd = p.imhd != 0;
-  const bool dust = p.idust != 0;
+  constexpr bool mhd = PHYS == 1;
+  constexpr bool dust = PHYS == 2;
   const PT ddq2 = (PT)p.ddq2table, dq2 = (PT)p.dq2table;
   const double R2 = p.radkern2;
   const PT beta = (PT)p.beta;
@@ -684,7 +1040,8 @@ __global__ __launch_bounds__(BS) void k_force(KArgs a, Phys p, Grid g) {
     divb.zero(); cbx.zero(); cby.zero(); cbz.zero(); gpx.zero(); gpy.zero(); gpz.zero();
     del2u.zero(); deps.zero();
 
-    if (!err) for_each_candidate<NDIM>(a, g, xi, rad, [&](int kk, const double* xj, const int* flip, bool img) {
+    const bool uselist = p.use_list && a.nbc[k] >= 0 && !a.sc->list_invalid;
+    auto body = [&](int kk, const double* xj, const int* flip, int code) {
       double dx[3] = {0.0, 0.0, 0.0};
       for (int d = 0; d < NDIM; d++) dx[d] = xi[d] - xj[d];
       const double rij2 = rij2_exact<NDIM>(dx);
@@ -692,7 +1049,7 @@ __global__ __launch_bounds__(BS) void k_force(KArgs a, Phys p, Grid g) {
       const double hj1 = a.P4[kk].w;
       const double q2j = mul_exact(mul_exact(rij2, hj1), hj1);
       if (!(q2i < R2 || q2j < R2)) return;
-      if (kk == k && !img) return;
+      if (kk == k && code == 0) return;
       npairs++;
       if (rij2 <= 0.0) { err = NDSP_ERR_COINCIDENT; return; }
       const PT rij = sqrt((PT)rij2);
@@ -849,7 +1206,11 @@ __global__ __launch_bounds__(BS) void k_force(KArgs a, Phys p, Grid g) {
       fz.add(f[2]);
       dudt.add(du);
       if (p.iavlim[1] > 0) del2u.add(mj * rho1j * ((uui - j2.x) * rinv) * grkerni);
-    });
+    };
+    if (!err) {
+      if (uselist) for_each_listed<NDIM>(a, g, k, a.nbc[k] + a.nbx[k], body);
+      else for_each_candidate<NDIM>(a, g, xi, rad, body);
+    }
     a.f[F_FX][i] = fx.value();
     a.f[F_FY][i] = fy.value();
     a.f[F_FZ][i] = fz.value();
@@ -867,17 +1228,12 @@ __global__ __launch_bounds__(BS) void k_force(KArgs a, Phys p, Grid g) {
     a.f[F_DEL2U][i] = del2u.value();
     a.f[F_DEPS][i] = deps.value();
   }
-  // one atomic per wavefront for the global extrema and counters
-  PT vm = wave_max(vsigmax_l);
-  double dm = wave_min(dtc_l);
-  unsigned long long np = npairs;
-  for (int o = 32; o > 0; o >>= 1) np += __shfl_xor(np, o);
-  if ((threadIdx.x & 63) == 0) {
-    atomic_max_pos(&a.sc->vsigmax, (double)vm);
-    atomic_min_pos(&a.sc->dtcourant, dm);
-    atomicAdd(&a.sc->pairs_force, np);
-  }
   if (err) atomicCAS(&a.sc->err, 0, err);
+  {
+    double rv[3] = {(double)vsigmax_l, dtc_l, (double)npairs};
+    const int rk[3] = {RED_MAX, RED_MIN, RED_SUM};
+    block_partials<3>(rv, rk, a.red, a.rcap);
+  }
 }
 
 // per-particle finalisation, src/ratesND_mhd.f90:482-943 (FP64)
@@ -962,12 +1318,9 @@ __global__ void k_finalize(KArgs a, Phys p) {
       f[F_DPSIDT][i] = 0.0;
     }
   }
-  double fm = wave_max(fh);
-  double dm = wave_min(dtd);
-  if ((threadIdx.x & 63) == 0) {
-    atomic_max_pos(&a.sc->fhmax, fm);
-    atomic_min_pos(&a.sc->dtdrag, dm);
-  }
+  double rv[2] = {fh, dtd};
+  const int rk[2] = {RED_MAX, RED_MIN};
+  block_partials<2>(rv, rk, a.red, a.rcap);
 }
 
 // timestep, src/ratesND_mhd.f90:938-943 and src/stepND_leapfrog_mhd.f90:252
@@ -987,6 +1340,7 @@ __global__ void k_init_density_scalars(DScal* sc, double bconstmax) {
   sc->hhmax_ghost = sc->hmax_bin;
   sc->ncalc_next = 0;
   sc->relink = 0;
+  sc->list_invalid = 0;
   sc->stressmax = fmax(0.0, bconstmax);
   sc->itsmax = 0;
   sc->nmin = INT_MAX;
@@ -1183,7 +1537,7 @@ __global__ void k_nb(KArgs a, Phys p, Grid g, int sym, const int* offs, int* out
   int cnt = 0;
   if (!(rad < p.boxmin)) { if (!offs) out[k] = 0; return; }
   int base = offs ? offs[k] : 0;
-  for_each_candidate<NDIM>(a, g, xi, rad, [&](int kk, const double* xj, const int* flip, bool img) {
+  for_each_candidate<NDIM>(a, g, xi, rad, [&](int kk, const double* xj, const int* flip, int icode) {
     double dx[3] = {0.0, 0.0, 0.0};
     for (int d = 0; d < NDIM; d++) dx[d] = xi[d] - xj[d];
     const double rij2 = rij2_exact<NDIM>(dx);
@@ -1194,7 +1548,7 @@ __global__ void k_nb(KArgs a, Phys p, Grid g, int sym, const int* offs, int* out
     if (!in) return;
     if (offs) {
       int code = 0;
-      if (img) {
+      if (icode) {
         const double4 pj = a.P4[kk];
         const double xo[3] = {pj.x, pj.y, pj.z};
         for (int d = NDIM - 1; d >= 0; d--) {
@@ -1310,6 +1664,7 @@ static double nexp_of(const Phys& p) {
 }
 
 struct ndsp_ctx {
+  bool list_on = false;  // neighbour lists built by the last binning
   Phys p;
   int prec;
   int n;
@@ -1428,13 +1783,16 @@ static int do_linklist(ndsp_ctx* c, int in_step) {
   {
     TScope t(c, "binstats");
     hipLaunchKernelGGL(k_binstats, dim3(c->nb_part), dim3(BS), 0, c->st, a, c->p.ndim, c->part);
-    hipLaunchKernelGGL(k_binstats_final, dim3(1), dim3(1), 0, c->st, a.sc, c->part, c->nb_part);
+    hipLaunchKernelGGL(k_binstats_final, dim3(1), dim3(BS), 0, c->st, a.sc, c->part, c->nb_part);
   }
   DScal hs;
   if (int e = sync_scalars(c, hs)) return e;
   const double hmax = hs.hmax_bin;
-  const double cs = c->p.radkern * hmax;
-  if (!(cs > 0.0) || !std::isfinite(cs)) return set_err(NDSP_ERR_CELLS, "link: max h <= 0");
+  const double rmax = c->p.radkern * hmax;
+  if (!(rmax > 0.0) || !std::isfinite(rmax)) return set_err(NDSP_ERR_CELLS, "link: max h <= 0");
+  // cells of side >= radkern*hmax/2: a search visits fewer candidates than
+  // with whole-radius cells once rows are cut to the sphere
+  const double cs = rmax / NDSP_CELL_SUB;
   Grid g = c->g;
   long long ncell = 1;
   for (int d = 0; d < 3; d++) {
@@ -1444,7 +1802,7 @@ static int do_linklist(ndsp_ctx* c, int in_step) {
     }
     if (g.bound[d] >= 2) {
       const double L = g.xmax[d] - g.xmin[d];
-      if (L < cs) return set_err(NDSP_ERR_CELLS, "link: radkern*hmax exceeds the box");
+      if (L < rmax) return set_err(NDSP_ERR_CELLS, "link: radkern*hmax exceeds the box");
       double ncd = std::floor(L / cs);
       if (ncd > 1.0e8) return set_err(NDSP_ERR_CELLS, "link: too many cells");
       g.nc[d] = std::max(1, (int)ncd);
@@ -1494,6 +1852,22 @@ static int do_linklist(ndsp_ctx* c, int in_step) {
     else
       hipLaunchKernelGGL(k_pack_density<float>, dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p);
   }
+  // lists need the minimum image to be unique: 2 radkern hmax (1+skin) < L
+  c->list_on = c->p.use_list != 0;
+  for (int d = 0; d < c->p.ndim; d++)
+    if (g.bound[d] == 3 && !(2.0 * rmax * (1.0 + c->p.skin) * 1.001 < g.xmax[d] - g.xmin[d]))
+      c->list_on = false;
+  if (c->list_on) {
+    TScope t(c, "nblist");
+    HIPCHK(hipMemsetAsync(&a.sc->list_overflow, 0, sizeof(int), c->st));
+    HIPCHK(hipMemsetAsync(a.nbx, 0, sizeof(int) * n, c->st));
+    dispatch_ndim(c->p.ndim, [&](auto nd) {
+      constexpr int ND = decltype(nd)::value;
+      hipLaunchKernelGGL((k_nblist<ND>), dim3(nblk(n, 64)), dim3(64), 0, c->st, a, c->p, g);
+      return 0;
+    });
+    hipLaunchKernelGGL(k_nblist_sort, dim3(nblk(n)), dim3(BS), 0, c->st, a);
+  }
   HIPCHK(hipGetLastError());
   c->grid_valid = true;
   return 0;
@@ -1515,14 +1889,21 @@ static int do_density(ndsp_ctx* c) {
     const int persistent = (!reflect && !sym) ? 1 : 0;
     dispatch_ndim(c->p.ndim, [&](auto nd) {
       constexpr int ND = decltype(nd)::value;
-      if (c->prec == NDSP_PREC_FP64)
-        hipLaunchKernelGGL((k_nr_pass<ND, double>), dim3(nblk(nlist)), dim3(BS), 0, c->st, a, c->p, c->g, list,
-                           nlist, its, sym, persistent, next);
-      else
-        hipLaunchKernelGGL((k_nr_pass<ND, float>), dim3(nblk(nlist)), dim3(BS), 0, c->st, a, c->p, c->g, list,
-                           nlist, its, sym, persistent, next);
+      const bool fp64 = c->prec == NDSP_PREC_FP64, dust = c->p.idust != 0;
+      auto kern = fp64 ? (dust ? k_nr_pass<ND, double, true> : k_nr_pass<ND, double, false>)
+                       : (dust ? k_nr_pass<ND, float, true> : k_nr_pass<ND, float, false>);
+      Phys pl = c->p;
+      pl.use_list = c->list_on ? 1 : 0;
+      hipLaunchKernelGGL(kern, dim3(nblk(nlist)), dim3(BS), 0, c->st, a, pl, c->g, list, nlist, its, sym,
+                         persistent, next);
       return 0;
     });
+    hipLaunchKernelGGL(k_red_final, dim3(1), dim3(BS), 0, c->st, (const double*)a.red, a.rcap, (int)nblk(nlist),
+                       red_spec({RED_Q(RED_MAX, RED_F64, stressmax), RED_Q(RED_MAX, RED_F64, hmax_nr),
+                                 RED_Q(RED_MAX, RED_I32, itsmax), RED_Q(RED_MIN, RED_I32, nmin),
+                                 RED_Q(RED_MAX, RED_I32, nmax), RED_Q(RED_SUM, RED_U64, pairs_density),
+                                 RED_Q(RED_SUM, RED_U64, ncalc_total)}),
+                       a.sc);
     HIPCHK(hipGetLastError());
     if (persistent) break;
     DScal hs;
@@ -1557,16 +1938,27 @@ static int do_rates(ndsp_ctx* c) {
     TScope t(c, "force");
     dispatch_ndim(c->p.ndim, [&](auto nd) {
       constexpr int ND = decltype(nd)::value;
+      const int phys = c->p.imhd ? 1 : c->p.idust ? 2 : 0;
+      void (*kern)(KArgs, Phys, Grid);
       if (c->prec == NDSP_PREC_FP64)
-        hipLaunchKernelGGL((k_force<ND, double>), dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p, c->g);
+        kern = phys == 1 ? k_force<ND, double, 1> : phys == 2 ? k_force<ND, double, 2> : k_force<ND, double, 0>;
       else
-        hipLaunchKernelGGL((k_force<ND, float>), dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p, c->g);
+        kern = phys == 1 ? k_force<ND, float, 1> : phys == 2 ? k_force<ND, float, 2> : k_force<ND, float, 0>;
+      Phys pl = c->p;
+      pl.use_list = c->list_on ? 1 : 0;
+      hipLaunchKernelGGL(kern, dim3(nblk(n)), dim3(BS), 0, c->st, a, pl, c->g);
       return 0;
     });
+    hipLaunchKernelGGL(k_red_final, dim3(1), dim3(BS), 0, c->st, (const double*)a.red, a.rcap, (int)nblk(n),
+                       red_spec({RED_Q(RED_MAX, RED_F64, vsigmax), RED_Q(RED_MIN, RED_F64, dtcourant),
+                                 RED_Q(RED_SUM, RED_U64, pairs_force)}),
+                       a.sc);
   }
   {
     TScope t(c, "finalize");
     hipLaunchKernelGGL(k_finalize, dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p);
+    hipLaunchKernelGGL(k_red_final, dim3(1), dim3(BS), 0, c->st, (const double*)a.red, a.rcap, (int)nblk(n),
+                       red_spec({RED_Q(RED_MAX, RED_F64, fhmax), RED_Q(RED_MIN, RED_F64, dtdrag)}), a.sc);
     hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, c->st, a.sc, c->p);
   }
   HIPCHK(hipGetLastError());
@@ -1669,6 +2061,12 @@ int ndsp_create(const int32_t* iopt, const double* dopt, int64_t npart, int32_t 
     if (ib[d] >= 2 && !(dopt[16 + d] > dopt[13 + d])) bad = "xmax must exceed xmin";
   }
   if (nrefl > 1 && bad.empty()) bad = "reflecting walls in at most one dimension";
+  // neighbour lists need ghost extents that do not change between the list
+  // build and its use, which holds unless a wall reflects
+  // (a list entry packs the sorted index in 25 bits and the image code in 7)
+  p.use_list = (nrefl == 0 && npart < (int64_t(1) << 25)) ? 1 : 0;
+  if (iopt[17] != 0) p.use_list = 0;
+  p.skin = 0.04;
   if (!bad.empty()) return set_err(NDSP_ERR_BAD_OPTION, bad);
   p.dndim = 1.0 / p.ndim;
   p.boxmin = DBL_MAX;
@@ -1730,6 +2128,21 @@ int ndsp_create(const int32_t* iopt, const double* dopt, int64_t npart, int32_t 
   ok &= hipMalloc(&a.G4, v4) == hipSuccess;
   ok &= hipMalloc(&a.G5, v1) == hipSuccess;
   ok &= hipMalloc((void**)&a.sc, sizeof(DScal)) == hipSuccess;
+  a.rcap = (int)nblk(n);
+  ok &= hipMalloc((void**)&a.red, sizeof(double) * RED_MAXQ * (size_t)a.rcap) == hipSuccess;
+  {
+    // list capacity: 1.8x the neighbour count of a uniform lattice within
+    // radkern*hfact*(1+skin) (a particle with more falls back to the scan)
+    const double r = p.radkern * p.hfact * (1.0 + p.skin);
+    const double vol = p.ndim == 3 ? 4.18879020478639 * r * r * r : p.ndim == 2 ? M_PI * r * r : 2.0 * r;
+    a.maxnb = ((int)(1.8 * vol + 8.0) + 31) / 32 * 32;
+  }
+  if (p.use_list) {
+    ok &= hipMalloc(&a.nbl, sizeof(unsigned) * n * (size_t)a.maxnb) == hipSuccess;
+    ok &= hipMalloc(&a.nbc, sizeof(int) * n) == hipSuccess;
+    ok &= hipMalloc(&a.nbx, sizeof(int) * n) == hipSuccess;
+    ok &= hipMalloc(&a.hcap, sizeof(double) * n) == hipSuccess;
+  }
   float* tabf = nullptr;
   double* tabd = nullptr;
   ok &= hipMalloc(&tabf, sizeof(float) * tab.size()) == hipSuccess;
@@ -1763,7 +2176,8 @@ int ndsp_destroy(ndsp_ctx* c) {
   for (int fi = 0; fi < F_NFIELDS; fi++) if (a.f[fi]) hipFree(a.f[fi]);
   void* ptrs[] = {c->scratch_d, c->scratch_i, a.numneigh, a.nits, a.uid, a.cellid, a.sidx, a.P4, a.dxb,
                   a.D4, a.De, a.G0, a.G1, a.G2, a.G3, a.G4, a.G5, a.sc, (void*)a.tabf, (void*)a.tabd,
-                  c->part, c->cons_out, a.cstart, a.cfill, c->bsum, c->list_a, c->list_b};
+                  c->part, c->cons_out, a.cstart, a.cfill, c->bsum, c->list_a, c->list_b,
+                  a.nbl, a.nbc, a.nbx, a.hcap, a.red};
   for (void* q : ptrs) if (q) hipFree(q);
   if (c->mlp_alloced) mlp_free(c->mlp);
   for (auto e : c->ev0) hipEventDestroy(e);

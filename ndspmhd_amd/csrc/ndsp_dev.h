@@ -5,11 +5,13 @@
 #include <stdint.h>
 
 #define NDSP_IKERN 4000
+// cells per search radius radkern*hmax along each dimension
+#define NDSP_CELL_SUB 2
 
 // Cell grid.  Dimensions with walls (periodic or reflecting) are tiled exactly
-// by nc cells of width w = (xmax - xmin)/nc >= radkern*hmax, so a periodic or
-// mirror image of a cell is again a whole cell; open dimensions tile the
-// particle bounding box.
+// by nc cells of width w = (xmax - xmin)/nc >= radkern*hmax/NDSP_CELL_SUB, so
+// a periodic or mirror image of a cell is again a whole cell; open dimensions
+// tile the particle bounding box.
 struct Grid {
   int ndim;
   int nc[3];
@@ -93,6 +95,14 @@ __device__ __forceinline__ void cell_range(const Grid& g, int d, double x, doubl
     if (lo < -g.nc[d]) lo = -g.nc[d];
     if (hi > 2 * g.nc[d] - 1) hi = 2 * g.nc[d] - 1;
   }
+}
+
+// distance along d from x to unwrapped cell c, less a margin of 1e-9 cell
+// widths (0 when x lies within the widened cell)
+__device__ __forceinline__ double cell_gap(const Grid& g, int d, int c, double x) {
+  const double a = g.lo[d] + c * g.w[d], b = a + g.w[d];
+  const double gap = fmax(fmax(a - x, x - b), 0.0);
+  return fmax(gap - 1.0e-9 * g.w[d], 0.0);
 }
 
 // FP64 squared separation with the reference's operation order and no FMA

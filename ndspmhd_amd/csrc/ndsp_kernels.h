@@ -34,6 +34,8 @@ struct Phys {
   double radkern, radkern2, dq2table, ddq2table;
   double Bconstmax;
   double boxmin;  // smallest wall-to-wall length (search radii must stay below it)
+  int use_list;   // neighbour lists (no reflecting walls)
+  double skin;    // relative h margin of the lists
 };
 
 // device scalars (one per context); doubles touched by atomics hold
@@ -57,6 +59,8 @@ struct DScal {
   int nmin, nmax;
   int ncalc_next;      // unconverged particles after a pass
   int relink;
+  int list_invalid;    // some h grew past its list's margin: force pass scans cells
+  int list_overflow;   // particles whose list did not fit (they scan cells)
 };
 
 template <typename PT>
@@ -88,6 +92,27 @@ struct KArgs {
   // kernel tables (FP32 and FP64 copies of W, dW, d2W)
   const float* tabf;  // 3 x (IKERN+1)
   const double* tabd;
+  // neighbour lists, particle-major: entry m of sorted particle k at
+  // nbl[k*maxnb + m] = (sorted index << 7) | image code (kinds kx + 5 ky + 25 kz)
+  unsigned* nbl;
+  int* nbc;           // gather-list length, -1 when the list overflowed
+  int* nbx;           // transposed extras after the gather list
+  double* hcap;       // the list holds every neighbour while h <= hcap
+  int maxnb;
+  // per-block partials of the global reductions: quantity q of block b at
+  // red[q * rcap + b] (combined into sc by k_red_final)
+  double* red;
+  int rcap;
   DScal* sc;
   int n;
+};
+
+// one global reduction: op | type << 4, into the DScal field at byte offset off
+enum { RED_SUM = 0, RED_MAX = 1, RED_MIN = 2 };
+enum { RED_F64 = 0, RED_I32 = 1, RED_U64 = 2 };
+#define RED_MAXQ 8
+struct RedSpec {
+  int nq;
+  int op[RED_MAXQ];
+  int off[RED_MAXQ];
 };

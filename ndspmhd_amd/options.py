@@ -18,7 +18,7 @@ import numpy as np
 IOPT = dict(ikernel=0, iener=1, imhd=2, imagforce=3, idivbzero=4, iav=5,
             iavlim0=6, iavlim1=7, iavlim2=8, idust=9, idrag_nature=10,
             ibound0=11, ibound1=12, ibound2=13, maxdensits=14, idustevol=15,
-            ndim=16)
+            ndim=16, cellscan=17)
 DOPT = dict(hfact=0, tolh=1, gamma=2, polyk=3, Kdrag=4, alphamin=5,
             alphaumin=6, alphaBmin=7, beta=8, avdecayconst=9,
             psidecayfact=10, C_cour=11, C_force=12, xmin0=13, xmin1=14,
@@ -62,6 +62,9 @@ class Options:
     psep: float = 0.01
     Bconst: tuple = (0.0, 0.0, 0.0)
     dt: float = 0.0
+    # device-only: 1 = scan the cell grid in every pass instead of reusing
+    # per-particle neighbour lists (results are identical; lists are faster)
+    cellscan: int = 0
 
     def replace(self, **kw) -> "Options":
         return dataclasses.replace(self, **kw)
@@ -73,7 +76,8 @@ class Options:
     def iopt(self) -> np.ndarray:
         a = np.zeros(32, dtype=np.int32)
         for k in ("ikernel", "iener", "imhd", "imagforce", "idivbzero", "iav",
-                  "idust", "idrag_nature", "maxdensits", "idustevol", "ndim"):
+                  "idust", "idrag_nature", "maxdensits", "idustevol", "ndim",
+                  "cellscan"):
             a[IOPT[k]] = getattr(self, k)
         for d in range(3):
             a[IOPT["iavlim%d" % d]] = self.iavlim[d]

@@ -1,0 +1,80 @@
+"""Neighbour lists against the per-pass cell scan on the GPU.
+
+Every pair taken from a list still passes the same exact FP64 tests where it
+is used, so both paths sum the same pairs; only the order differs for the
+transposed part of a list, so the states agree to round-off (norm-relative
+1e-12 in FP64; 1e-6 mixed, where psi, driven by the cancelling divB sums,
+picks up order differences of ~2e-8 over four steps), and neighbour counts
+and iteration counts are identical.  Also
+covered: an evaluation whose h grows beyond the list's skin (the list is then
+abandoned, with the same results)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("x", "vel", "Bevol", "en", "hh", "alpha", "psi", "dustevol", "rho",
+          "force", "dBevoldt", "dudt", "ddustevoldt")
+
+
+def _pair(o, st, prec, nsteps):
+    import ndspmhd_amd as nd
+    out = []
+    for cellscan in (0, 1):
+        sim = nd.Simulation(o.replace(cellscan=cellscan), st, precision=prec)
+        sim.set_timing(True)
+        sim.derivs()
+        names = set(sim.kernel_times())
+        counts = (sim.get("numneigh").copy(), sim.get("nits").copy())
+        for _ in range(nsteps):
+            sim.step()
+        out.append((sim, names, counts))
+    return out
+
+
+def _close(a, b, tol):
+    for k in FIELDS:
+        x, y = a.get(k), b.get(k)
+        err = np.max(np.abs(x - y)) / max(np.max(np.abs(y)), 1e-300)
+        assert err <= tol, (k, err)
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-12), (0, 1e-6)])
+def test_mhd_box_lists_match_scan(prec, tol):
+    from ndspmhd_amd.setups import periodic_mhd_box
+    o, st = periodic_mhd_box(12, seed=3, vel_seed=4, vrms=0.3)
+    # uneven h, so that the transposed part of the lists is not empty
+    rng = np.random.default_rng(8)
+    st["hh"] *= np.exp(0.1 * rng.standard_normal(st["hh"].shape))
+    print("mhd box 12^3 lists vs cell scan prec", prec, flush=True)
+    (s0, n0, c0), (s1, n1, c1) = _pair(o, st, prec, 4)
+    assert "nblist" in n0 and "nblist" not in n1
+    assert np.array_equal(c0[0], c1[0]) and np.array_equal(c0[1], c1[1])
+    _close(s0, s1, tol)
+
+
+@pytest.mark.parametrize("ndim,nside", [(2, 24), (1, 200)])
+def test_dust_box_lists_match_scan(ndim, nside):
+    from ndspmhd_amd.setups import dusty_box
+    o, st = dusty_box(nside, ndim=ndim, seed=9)
+    print("dust box lists vs cell scan ndim", ndim, flush=True)
+    (s0, n0, c0), (s1, n1, c1) = _pair(o, st, 0, 3)
+    assert "nblist" in n0
+    assert np.array_equal(c0[0], c1[0])
+    _close(s0, s1, 1e-6)
+
+
+def test_list_abandoned_when_h_outgrows_skin():
+    """h seeded 30% below its converged value: the iteration grows h beyond
+    the 4% skin, the force pass must fall back to the cell scan."""
+    from ndspmhd_amd.setups import periodic_mhd_box
+    o, st = periodic_mhd_box(10, seed=5, vel_seed=6)
+    st["hh"] *= 0.7
+    import ndspmhd_amd as nd
+    res = []
+    for cellscan in (0, 1):
+        sim = nd.Simulation(o.replace(cellscan=cellscan), st, precision=1)
+        sim.derivs()
+        res.append(sim)
+    assert np.array_equal(res[0].get("numneigh"), res[1].get("numneigh"))
+    _close(res[0], res[1], 1e-12)
