@@ -52,10 +52,11 @@ def setup(nside):
     return _SETUPS[nside]
 
 
-def run_gpu(nside, steps, warmup, prec, timing=True):
+def run_gpu(nside, steps, warmup, prec, timing=True, graph=False):
     import ndspmhd_amd as nd
     o, st = setup(nside)
     sim = nd.Simulation(o, st, precision=prec)
+    sim.set_graph(graph)
     sim.derivs()
     for _ in range(warmup):
         sim.step()
@@ -132,7 +133,6 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nside", type=int, default=128)
-    ap.add_argument("--fp64-steps", type=int, default=5)
     ap.add_argument("--cpu-nside", type=int, default=96)
     ap.add_argument("--big-nside", type=int, default=256,
                     help="second workload (mixed vs plain FP64 steps/s); 0 skips it")
@@ -195,12 +195,19 @@ def main():
         "ncalc_mean": r["ncalc_mean"],
     }
     r["sim"].close()
+    log("[bench] same workload, steps replayed as HIP graphs")
+    rg = run_gpu(args.nside, args.steps, args.warmup, prec=0, timing=False, graph=True)
+    result["graph_replay"] = {"steps_per_s": args.steps / rg["wall"],
+                              "pairs_per_s": (rg["pairs_density"] + rg["pairs_force"]) / rg["wall"],
+                              "ms_per_step": 1e3 * rg["wall"] / args.steps}
+    rg["sim"].close()
     if not args.no_fp64:
-        log("[bench] plain FP64 GPU, %d steps" % args.fp64_steps)
-        r64 = run_gpu(args.nside, args.fp64_steps, 1, prec=1, timing=False)
+        log("[bench] plain FP64 GPU, same steps, graph replay")
+        r64 = run_gpu(args.nside, args.steps, args.warmup, prec=1, timing=False, graph=True)
         v64 = (r64["pairs_density"] + r64["pairs_force"]) / r64["wall"]
-        result["fp64_gpu"] = {"value": v64, "ms_per_step": 1e3 * r64["wall"] / args.fp64_steps,
-                              "speedup_mixed_over_fp64": value / v64}
+        result["fp64_gpu"] = {"value": v64, "ms_per_step": 1e3 * r64["wall"] / args.steps,
+                              "steps_per_s": args.steps / r64["wall"],
+                              "speedup_mixed_over_fp64": (args.steps / rg["wall"]) / (args.steps / r64["wall"])}
         r64["sim"].close()
         result["error_vs_fp64"] = field_errors(min(args.nside, 64))
     if args.big_nside > 0 and not args.no_fp64:
@@ -208,7 +215,7 @@ def main():
         log("[bench] %d^3 mixed vs plain FP64, %d steps each" % (nb, args.big_steps))
         big = {}
         for prec, key in ((0, "mixed"), (1, "fp64")):
-            rb = run_gpu(nb, args.big_steps, 1, prec=prec, timing=False)
+            rb = run_gpu(nb, args.big_steps, 1, prec=prec, timing=False, graph=True)
             big[key] = {"steps_per_s": args.big_steps / rb["wall"],
                         "pairs_per_s": (rb["pairs_density"] + rb["pairs_force"]) / rb["wall"],
                         "ncalc_mean": rb["ncalc_mean"]}

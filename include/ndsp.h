@@ -150,6 +150,14 @@ int ndsp_set_timing(ndsp_ctx *ctx, int32_t on);
 int ndsp_kernel_times(ndsp_ctx *ctx, double *ms, int32_t cap, int32_t *n,
                       char *names, int32_t names_cap);
 
+/* Step graphs (no counterpart in the reference, whose step is a host loop):
+ * with on != 0, ndsp_step records its two halves (predictor + binning +
+ * first h-rho passes; rates + corrector + next dt) into HIP graphs once and
+ * replays them, completing the h-rho iteration on the host path in between
+ * when it needs more passes than recorded.  Results are identical to the
+ * launch-by-launch path.  Not used while timing or the h predictor is on. */
+int ndsp_set_graph(ndsp_ctx *ctx, int32_t on);
+
 /* ---- h-predictor MLP (no counterpart in the reference; its seed for the
  * h-rho iteration is h = h_in + dt dh/dt, src/stepND_leapfrog_mhd.f90:153).
  * A 6 -> 32 -> 32 (GELU) -> 1 network, BF16 weights, FP32 accumulation,

@@ -34,7 +34,7 @@ CONSERVED = ("ekin", "etherm", "emag", "etot", "momx", "momy", "momz", "momtot",
 SYMBOLS = ("ndsp_create", "ndsp_destroy", "ndsp_upload", "ndsp_download",
            "ndsp_set_linklist", "ndsp_iterate_density", "ndsp_get_rates",
            "ndsp_derivs", "ndsp_step", "ndsp_set_dt", "ndsp_conserved",
-           "ndsp_neighbours", "ndsp_set_timing", "ndsp_kernel_times",
+           "ndsp_neighbours", "ndsp_set_timing", "ndsp_kernel_times", "ndsp_set_graph",
            "ndsp_mlp_config", "ndsp_mlp_set_params", "ndsp_mlp_get_params",
            "ndsp_mlp_forward", "ndsp_mlp_grad", "ndsp_mlp_train", "ndsp_last_error")
 MLP_NPARAM = 1313
@@ -81,6 +81,7 @@ def load_library(path: str | None = None):
     L.ndsp_conserved.argtypes = [vp, dp]
     L.ndsp_neighbours.argtypes = [vp, ctypes.c_int32, ip, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
     L.ndsp_set_timing.argtypes = [vp, ctypes.c_int32]
+    L.ndsp_set_graph.argtypes = [vp, ctypes.c_int32]
     L.ndsp_kernel_times.argtypes = [vp, dp, ctypes.c_int32, ip, ctypes.c_char_p, ctypes.c_int32]
     fp = ctypes.POINTER(ctypes.c_float)
     L.ndsp_mlp_config.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
@@ -237,6 +238,10 @@ class Simulation:
 
     def set_timing(self, on: bool):
         self._check(load_library().ndsp_set_timing(self.h, int(on)))
+
+    def set_graph(self, on: bool):
+        """replay each step as two recorded HIP graphs (ndsp_set_graph)"""
+        self._check(load_library().ndsp_set_graph(self.h, int(on)))
 
     def kernel_times(self) -> dict:
         L = load_library()

@@ -153,6 +153,74 @@ __global__ __launch_bounds__(BS) void k_binstats_final(DScal* sc, const double* 
   }
 }
 
+// Cell grid of this binning, on the device (no host round trip): cells of
+// side >= radkern*hmax/NDSP_CELL_SUB (src/linkND.f90:72 uses radkern*hhmax);
+// walls tile their dimension exactly; cells are widened when the Morton key
+// space would exceed 2^cap_bits.  Also decides whether neighbour lists can
+// be used (2*radkern*hmax*(1+skin) below every periodic length, so the
+// minimum image of a neighbour is unique).
+__global__ void k_grid_setup(DScal* sc, Grid* gd, Grid box, Phys p, int cap_bits) {
+  if (threadIdx.x | blockIdx.x) return;
+  Grid g = box;
+  const double hmax = sc->hmax_bin;
+  const double rmax = p.radkern * hmax;
+  sc->grid_clamped = 0;
+  if (!(rmax > 0.0) || !(rmax < DBL_MAX)) sc->err = NDSP_ERR_CELLS;
+  double cs = rmax / NDSP_CELL_SUB;
+  const int mbmax = cap_bits / p.ndim;
+  for (int d = 0; d < 3; d++) {
+    if (d >= p.ndim) {
+      g.nc[d] = 1; g.lo[d] = 0.0; g.w[d] = 1.0; g.inv_w[d] = 1.0;
+      continue;
+    }
+    double ncd;
+    if (g.bound[d] >= 2) {
+      const double L = g.xmax[d] - g.xmin[d];
+      if (!(L >= rmax)) sc->err = NDSP_ERR_CELLS;
+      ncd = floor(L / cs);
+      if (!(ncd >= 1.0)) ncd = 1.0;
+      if (ncd > (double)(1 << mbmax)) { ncd = (double)(1 << mbmax); sc->grid_clamped = 1; }
+      g.nc[d] = (int)ncd;
+      g.lo[d] = g.xmin[d];
+      g.w[d] = L / g.nc[d];
+    } else {
+      const double ext = sc->bbox_hi[d] - sc->bbox_lo[d];
+      ncd = floor(ext / cs) + 1.0;
+      double w = cs;
+      if (!(ncd <= (double)(1 << mbmax))) {
+        ncd = (double)(1 << mbmax);
+        w = fmax(cs, ext / (ncd - 1.0) * (1.0 + 1.0e-12));
+        sc->grid_clamped = 1;
+      }
+      g.nc[d] = (int)ncd;
+      g.lo[d] = sc->bbox_lo[d];
+      g.w[d] = w;
+    }
+    g.inv_w[d] = 1.0 / g.w[d];
+  }
+  int mx = max(g.nc[0], max(g.nc[1], g.nc[2]));
+  int mb = 0;
+  while ((1 << mb) < mx) mb++;
+  g.mbits = mb;
+  g.ncell = 1LL << (p.ndim * mb);
+  int on = p.use_list;
+  for (int d = 0; d < p.ndim; d++)
+    if (g.bound[d] == 3 && !(2.0 * rmax * (1.0 + p.skin) * 1.001 < g.xmax[d] - g.xmin[d])) on = 0;
+  sc->list_on = on;
+  sc->list_overflow = 0;
+  *gd = g;
+}
+
+__global__ void k_fill_cells(int* __restrict__ a, const Grid* __restrict__ gd) {
+  const long long m = gd->ncell + 1;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (long long)gridDim.x * blockDim.x)
+    a[i] = 0;
+}
+
+__global__ void k_fill_i(int* __restrict__ a, int n, int v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = v;
+}
+
 __device__ __forceinline__ int cell_coord(const Grid& g, int d, double x) {
   int c = (int)floor((x - g.lo[d]) * g.inv_w[d]);
   if (c < 0) c = 0;
@@ -160,20 +228,23 @@ __device__ __forceinline__ int cell_coord(const Grid& g, int d, double x) {
   return c;
 }
 
-__global__ void k_cellid(KArgs a, Grid g) {
+__global__ void k_cellid(KArgs a, const Grid* __restrict__ gp) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
+  const Grid g = *gp;
   int cx = cell_coord(g, 0, a.f[F_X][i]);
   int cy = g.ndim >= 2 ? cell_coord(g, 1, a.f[F_Y][i]) : 0;
   int cz = g.ndim >= 3 ? cell_coord(g, 2, a.f[F_Z][i]) : 0;
-  int c = cx + g.nc[0] * (cy + g.nc[1] * cz);
+  int c = cell_key(g, cx, cy, cz);
   a.cellid[i] = c;
   atomicAdd(&a.cfill[c], 1);
 }
 
 // exclusive scan, 3 phases (1024 elements per block)
-__global__ void k_scan_local(const int* in, int* out, int* bsum, long long n) {
+__global__ void k_scan_local(const int* in, int* out, int* bsum, const Grid* __restrict__ gp) {
   __shared__ int sm[BS];
+  const long long n = gp->ncell + 1;
+  if ((long long)blockIdx.x * BS * 4 >= n) return;
   const int ITEMS = 4;
   long long base = (long long)blockIdx.x * BS * ITEMS + (long long)threadIdx.x * ITEMS;
   int v[ITEMS];
@@ -200,8 +271,9 @@ __global__ void k_scan_local(const int* in, int* out, int* bsum, long long n) {
   if (threadIdx.x == BS - 1) bsum[blockIdx.x] = sm[BS - 1];
 }
 
-__global__ void k_scan_bsum(int* bsum, int nb, int* total_out) {
+__global__ void k_scan_bsum(int* bsum, const Grid* __restrict__ gp, int* total_out) {
   __shared__ int sm[BS];
+  const int nb = (int)((gp->ncell + 1 + BS * 4 - 1) / (BS * 4));
   __shared__ int carry;
   if (threadIdx.x == 0) carry = 0;
   __syncthreads();
@@ -224,7 +296,8 @@ __global__ void k_scan_bsum(int* bsum, int nb, int* total_out) {
   if (threadIdx.x == 0 && total_out) *total_out = carry;
 }
 
-__global__ void k_scan_add(int* out, const int* bsum, long long n) {
+__global__ void k_scan_add(int* out, const int* bsum, const Grid* __restrict__ gp) {
+  const long long n = gp->ncell + 1;
   long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx < n) out[idx] += bsum[idx / (BS * 4)];
 }
@@ -239,9 +312,9 @@ __global__ void k_scatter(KArgs a) {
 
 // make the order inside each cell ascending in slot index (deterministic,
 // and a stable sort by cell key overall)
-__global__ void k_cellsort(KArgs a, long long ncell) {
+__global__ void k_cellsort(KArgs a, const Grid* __restrict__ gp) {
   long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
+  if (c >= gp->ncell) return;
   int b = a.cstart[c], e = a.cstart[c + 1];
   for (int k = b + 1; k < e; k++) {
     int v = a.sidx[k];
@@ -331,18 +404,6 @@ __device__ __forceinline__ void for_each_candidate(const KArgs& a, const Grid& g
       }
       int lx, hx;
       cell_range(g, 0, xi[0], NDIM >= 2 ? sqrt(r2) : rad, lx, hx);
-      const int rowbase = g.nc[0] * (sy + g.nc[1] * sz);
-      const int flip[3] = {0, ky >= 3, kz >= 3};
-      if (lx >= 0 && hx < g.nc[0] && kz == 0 && ky == 0) {
-        // contiguous run of cells with no image: one particle range
-        const int b = a.cstart[rowbase + lx], e = a.cstart[rowbase + hx + 1];
-        for (int kk = b; kk < e; kk++) {
-          const double4 pj = a.P4[kk];
-          const double xj[3] = {pj.x, pj.y, pj.z};
-          body(kk, xj, flip, 0);
-        }
-        continue;
-      }
       for (int cx = lx; cx <= hx; cx++) {
         int kx = 0;
         const int sx = cell_image(g, 0, cx, kx);
@@ -350,7 +411,8 @@ __device__ __forceinline__ void for_each_candidate(const KArgs& a, const Grid& g
         const int kind[3] = {kx, ky, kz};
         const int code = kx + 5 * ky + 25 * kz;
         const int fl[3] = {kx >= 3, ky >= 3, kz >= 3};
-        const int b = a.cstart[rowbase + sx], e = a.cstart[rowbase + sx + 1];
+        const int key = cell_key(g, sx, sy, sz);
+        const int b = a.cstart[key], e = a.cstart[key + 1];
         for (int kk = b; kk < e; kk++) {
           const double4 pj = a.P4[kk];
           double xj[3] = {pj.x, pj.y, pj.z};
@@ -366,36 +428,79 @@ __device__ __forceinline__ void for_each_candidate(const KArgs& a, const Grid& g
   }
 }
 
-// Same contract as for_each_candidate, over the first cnt entries of the
-// neighbour list of sorted particle k; image coordinates are recomputed with
-// the ghost arithmetic, so the exact FP64 tests downstream see the same values.
-template <int NDIM, typename F>
-__device__ __forceinline__ void for_each_listed(const KArgs& a, const Grid& g, int k, int cnt, F&& body) {
-  // particle-major lists, four entries per load
-  const uint4* L = (const uint4*)(a.nbl + (size_t)k * a.maxnb);
-  for (int m0 = 0; m0 < cnt; m0 += 4) {
-    const uint4 e4 = L[m0 >> 2];
-    const unsigned ev[4] = {e4.x, e4.y, e4.z, e4.w};
-    const int mm = min(4, cnt - m0);
-    for (int u = 0; u < mm; u++) {
-      const unsigned e = ev[u];
-      const int kk = (int)(e >> 7);
-      const int code = (int)(e & 127u);
-      const double4 pj = a.P4[kk];
-      double xj[3] = {pj.x, pj.y, pj.z};
-      int flip[3] = {0, 0, 0};
-      bool ok = true;
-      if (code) {
-        const int kind[3] = {code % 5, (code / 5) % 5, code / 25};
-        for (int d = 0; d < NDIM; d++) {
-          if (!kind[d]) continue;
-          ok = ok && image_coord(g, d, kind[d], xj[d], a.dxb[kk], xj[d]);
-          flip[d] = kind[d] >= 3;
+// Image code of a list entry: 2 bits per dimension, kind 0 (in place),
+// 1 (periodic image left of xmin) or 2 (right of xmax); lists exist only
+// without reflecting walls, so no other kind occurs.
+__device__ __forceinline__ unsigned list_code(int kx, int ky, int kz) {
+  return (unsigned)(kx | (ky << 2) | (kz << 4));
+}
+// the code of k seen from j, given the code of j seen from k (1 <-> 2)
+__device__ __forceinline__ unsigned inverse_code(unsigned code) {
+  return ((code & 0x15u) << 1) | ((code & 0x2Au) >> 1);
+}
+
+// Iterates the first cnt entries of the neighbour list of sorted particle k,
+// calling body(kk, r, xj_img, flip, code) with r = load(kk) (a struct holding
+// the FP64 position and 1/h in r.p and whatever the pair sum reads) and the
+// image coordinates of the ghost arithmetic (so the exact FP64 tests see the
+// values of the cell scan).  Entries come four per load; the record of the
+// next entry is fetched while the current one is processed, so a wavefront's
+// gathers overlap its arithmetic.
+template <int NDIM, typename LOAD, typename F>
+__device__ __forceinline__ void for_each_listed_pf(const KArgs& a, const Grid& g, int k, int cnt, LOAD&& load,
+                                                   F&& body) {
+  if (cnt <= 0) return;
+  const uint4* L4 = (const uint4*)(a.nbl + (size_t)k * a.maxnb);
+  const int ng = (cnt + 3) >> 2;
+  uint4 gc = L4[0];
+  auto r = load((int)(gc.x >> 7));
+  for (int gi = 0; gi < ng; gi++) {
+    const uint4 gn = gi + 1 < ng ? L4[gi + 1] : gc;
+    const unsigned ev[4] = {gc.x, gc.y, gc.z, gc.w};
+    const int mm = min(4, cnt - 4 * gi);
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (u < mm) {
+        const unsigned e = ev[u];
+        const bool more = 4 * gi + u + 1 < cnt;
+        const unsigned en = !more ? e : u < 3 ? ev[u + 1] : gn.x;
+        const auto rn = load((int)(en >> 7));
+        const int kk = (int)(e >> 7);
+        const int code = (int)(e & 63u);
+        double xj[3] = {r.p.x, r.p.y, r.p.z};
+        const int flip[3] = {0, 0, 0};
+        bool ok = true;
+        if (code) {
+          // ghost arithmetic of src/ghostND_mhd.f90:180-200; a periodic image
+          // of a particle exactly on the wall does not exist (:172)
+          for (int d = 0; d < NDIM; d++) {
+            const int kd = (code >> (2 * d)) & 3;
+            const double x0 = xj[d];
+            if (kd == 1) {
+              ok = ok && g.xmax[d] - x0 > 0.0;
+              xj[d] = g.xmin[d] + (x0 - g.xmax[d]);
+            } else if (kd == 2) {
+              ok = ok && x0 - g.xmin[d] > 0.0;
+              xj[d] = g.xmax[d] + (x0 - g.xmin[d]);
+            }
+          }
         }
+        if (ok) body(kk, r, xj, flip, code);
+        r = rn;
       }
-      if (ok) body(kk, xj, flip, code);
     }
+    gc = gn;
   }
+}
+
+// cell-scan counterpart of for_each_listed_pf (record loaded in place)
+template <int NDIM, typename LOAD, typename F>
+__device__ __forceinline__ void for_each_candidate_rec(const KArgs& a, const Grid& g, const double* xi,
+                                                       double rad, LOAD&& load, F&& body) {
+  for_each_candidate<NDIM>(a, g, xi, rad, [&](int kk, const double* xj, const int* flip, int code) {
+    const auto r = load(kk);
+    body(kk, r, xj, flip, code);
+  });
 }
 
 // Neighbour lists (periodic or open domains).  Entry m of sorted particle k
@@ -435,35 +540,39 @@ __device__ __forceinline__ int wave_max_i(int v) {
   return v;
 }
 
-// image code of k seen from j, given the code of j seen from k (periodic
-// kinds 1 <-> 2 per dimension)
-__device__ __forceinline__ unsigned inverse_code(unsigned code) {
-  unsigned r = 0, m = 1;
-  for (int d = 0; d < 3; d++) {
-    const unsigned kd = code % 5;
-    code /= 5;
-    r += m * (kd == 1 ? 2u : kd == 2 ? 1u : kd);
-    m *= 5;
-  }
-  return r;
-}
-
-// One wavefront per 64 consecutive sorted particles.  The wave walks the
-// union of its particles' gather boxes row by row (rows (cy, cz) in the
-// nested order of for_each_candidate, so each list keeps the scan's order);
-// a row segment of one image kind is a contiguous particle range, staged 64
-// candidates at a time in LDS.  Each lane first marks the candidates within
-// the list radius by an FP32 test on FP64 separations (relative margin 1e-4),
-// then applies the exact FP64 minimum-image test to the marked ones.  A pair
-// k-j with j in G(k) and k not in G(j) is appended to the extras of j, which
-// fill j's row from its end (k_nblist_sort moves them into place).
+// One wavefront per 64 consecutive sorted particles (a compact group: the
+// order is by Morton cell key).  The wave enumerates the cells of the union
+// of its particles' gather boxes (rows (cy, cz) cut to the union of the
+// particles' chords), in the nested order of for_each_candidate, so each
+// list keeps the scan's candidate order.  For a batch of up to NBL_CELLS
+// cells it loads all cell ranges at once, forms their prefix sums in LDS,
+// and then streams the batch's candidates 64 at a time through LDS (the next
+// 64 are fetched while the current ones are tested): FP64 raw positions for
+// the exact test and FP32 image positions relative to the wave's first
+// particle for a prefilter (relative margin 1e-4 on r^2, far above the FP32
+// rounding of the separations within a compact group).  Each lane marks the
+// candidates that pass the prefilter, then applies the exact FP64
+// minimum-image test to those.  A pair k-j with j in G(k) and k not in G(j) is
+// appended to the extras of j, which fill j's row from its end
+// (k_nblist_sort moves them into place).
+#define NBL_CELLS 256
 template <int NDIM>
-__global__ __launch_bounds__(64) void k_nblist(KArgs a, Phys p, Grid g) {
-  __shared__ double4 sx[64];
+__global__ __launch_bounds__(64) void k_nblist(KArgs a, Phys p, const Grid* __restrict__ gp) {
+  __shared__ double4 sraw[64];
+  __shared__ float4 srel[64];
   __shared__ unsigned se[64];
+  __shared__ int ckey[NBL_CELLS];
+  __shared__ int ccode[NBL_CELLS];
+  __shared__ int cbeg[NBL_CELLS];
+  __shared__ int cpre[NBL_CELLS + 1];
   const int lane = threadIdx.x;
-  const int k = blockIdx.x * 64 + lane;
+  const int k = xcd_tile() * 64 + lane;
   const bool act = k < a.n;
+  if (!a.sc->list_on) {
+    if (act) a.nbc[k] = -1;
+    return;
+  }
+  const Grid g = *gp;
   const double s1 = 1.0 + p.skin;
   const double lim2 = p.radkern2 * s1 * s1 * (1.0 + 1.0e-12);
   const float lim2f = (float)(p.radkern2 * s1 * s1 * (1.0 + 1.0e-4));
@@ -473,7 +582,107 @@ __global__ __launch_bounds__(64) void k_nblist(KArgs a, Phys p, Grid g) {
   const float hi2f = (float)hi21;
   const double rad = act ? p.radkern * s1 * (1.0 + 1.0e-9) / pk.w : 0.0;
   const double rad2 = rad * rad * (1.0 + 1.0e-12);
+  const double org[3] = {__shfl(xi[0], 0), __shfl(xi[1], 0), __shfl(xi[2], 0)};
+  const float xr0 = (float)(xi[0] - org[0]);
+  const float xr1 = NDIM >= 2 ? (float)(xi[1] - org[1]) : 0.f;
+  const float xr2 = NDIM >= 3 ? (float)(xi[2] - org[2]) : 0.f;
   unsigned* L = a.nbl + (size_t)(act ? k : 0) * a.maxnb;
+  int cnt = 0;
+  bool over = false;
+
+  // test the staged candidates [0, m) against this lane's particle
+  auto test = [&](int m) {
+    if (!act) return;
+    unsigned long long mask = 0ull;
+    for (int t = 0; t < m; t++) {
+      const float4 c = srel[t];
+      const float dx = xr0 - c.x, dy = xr1 - c.y, dz = xr2 - c.z;
+      const float r2f = dx * dx + dy * dy + dz * dz;
+      if (r2f * hi2f < lim2f) mask |= 1ull << t;
+    }
+    while (mask) {
+      const int t = __builtin_ctzll(mask);
+      mask &= mask - 1ull;
+      const double4 c = sraw[t];
+      const double xr[3] = {c.x, c.y, c.z};
+      const double rr = min_image_r2<NDIM>(g, xi, xr);
+      if (!(rr * hi21 < lim2)) continue;
+      const unsigned e = se[t];
+      if (cnt < a.maxnb) L[cnt] = e;
+      else over = true;
+      cnt++;
+      const int kk = (int)(e >> 7);
+      // same operands as kk's own test of k (rr is symmetric)
+      if (kk != k && !(rr * (c.w * c.w) < lim2)) {
+        const int slot = atomicAdd(&a.nbx[kk], 1);
+        if (slot < a.maxnb)
+          a.nbl[(size_t)kk * a.maxnb + (a.maxnb - 1 - slot)] = ((unsigned)k << 7) | inverse_code(e & 63u);
+      }
+    }
+  };
+
+  int ncl = 0;  // cells in the current batch (wave-uniform)
+  auto process_batch = [&]() {
+    __syncthreads();
+    for (int c = lane; c < ncl; c += 64) {
+      const int key = ckey[c];
+      const int b = key >= 0 ? a.cstart[key] : 0;
+      const int e = key >= 0 ? a.cstart[key + 1] : 0;
+      cbeg[c] = b;
+      cpre[c] = e - b;
+    }
+    __syncthreads();
+    int run = 0;
+    for (int c0 = 0; c0 < ncl; c0 += 64) {
+      const int v = c0 + lane < ncl ? cpre[c0 + lane] : 0;
+      int x = v;
+      for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+      }
+      if (c0 + lane < ncl) cpre[c0 + lane] = run + x - v;
+      run += __shfl(x, 63);
+    }
+    if (lane == 0) cpre[ncl] = run;
+    __syncthreads();
+    const int T = run;
+    // candidate q of the batch -> (raw record, list entry)
+    auto fetch = [&](int q, double4& pj, unsigned& ent) {
+      int lo = 0, hi = ncl - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cpre[mid] <= q) lo = mid;
+        else hi = mid - 1;
+      }
+      const int kk = cbeg[lo] + (q - cpre[lo]);
+      ent = ((unsigned)kk << 7) | (unsigned)ccode[lo];
+      pj = a.P4[kk];
+    };
+    double4 pj = make_double4(0.0, 0.0, 0.0, 0.0);
+    unsigned ent = 0u;
+    if (lane < T) fetch(lane, pj, ent);
+    for (int q0 = 0; q0 < T; q0 += 64) {
+      const int m = min(64, T - q0);
+      if (lane < m) {
+        const int code = (int)(ent & 63u);
+        double xj[3] = {pj.x, pj.y, pj.z};
+        for (int d = 0; d < NDIM; d++) {
+          const int kd = (code >> (2 * d)) & 3;
+          if (kd) image_coord(g, d, kd, xj[d], 0.0, xj[d]);
+        }
+        sraw[lane] = pj;
+        srel[lane] = make_float4((float)(xj[0] - org[0]), NDIM >= 2 ? (float)(xj[1] - org[1]) : 0.f,
+                                 NDIM >= 3 ? (float)(xj[2] - org[2]) : 0.f, 0.f);
+        se[lane] = ent;
+      }
+      __syncthreads();
+      if (q0 + 64 + lane < T) fetch(q0 + 64 + lane, pj, ent);  // next 64, in flight during the test
+      test(m);
+      __syncthreads();
+    }
+    ncl = 0;
+  };
+
   int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
   for (int d = 1; d < NDIM; d++) {
     int l = INT_MAX, h = INT_MIN;
@@ -481,8 +690,6 @@ __global__ __launch_bounds__(64) void k_nblist(KArgs a, Phys p, Grid g) {
     lo[d] = wave_min_i(l);
     hi[d] = wave_max_i(h);
   }
-  int cnt = 0;
-  bool over = false;
   for (int cz = lo[2]; cz <= hi[2]; cz++) {
     int kz = 0, sz = 0;
     double gz2 = 0.0;
@@ -505,69 +712,20 @@ __global__ __launch_bounds__(64) void k_nblist(KArgs a, Phys p, Grid g) {
       if (act && r2 >= 0.0) cell_range(g, 0, xi[0], NDIM >= 2 ? sqrt(r2) : rad, lx, hx);
       lx = wave_min_i(lx);
       hx = wave_max_i(hx);
-      if (lx > hx) continue;
-      const int rowbase = g.nc[0] * (sy + g.nc[1] * sz);
-      const int nx = g.nc[0];
-      for (int sgi = 0; sgi < 3; sgi++) {
-        const int c0 = sgi == 0 ? lx : sgi == 1 ? max(lx, 0) : max(lx, nx);
-        const int c1 = sgi == 0 ? min(hx, -1) : sgi == 1 ? min(hx, nx - 1) : hx;
-        if (c0 > c1) continue;
-        int kx = 0, kx1 = 0;
-        const int s0 = cell_image(g, 0, c0, kx);
-        const int s1c = cell_image(g, 0, c1, kx1);
-        if (s0 < 0 || s1c < 0) continue;
-        const int kind[3] = {kx, ky, kz};
-        const unsigned code = (unsigned)(kx + 5 * ky + 25 * kz);
-        // this lane's position shifted by minus the image offset of the segment
-        double xs[3] = {xi[0], xi[1], xi[2]};
-        for (int d = 0; d < NDIM; d++) {
-          const double Ld = g.xmax[d] - g.xmin[d];
-          if (kind[d] == 1) xs[d] = xi[d] + Ld;
-          else if (kind[d] == 2) xs[d] = xi[d] - Ld;
+      for (int cx0 = lx; cx0 <= hx; cx0 += 64) {
+        const int nadd = min(64, hx - cx0 + 1);
+        if (ncl + nadd > NBL_CELLS) process_batch();
+        if (lane < nadd) {
+          int kx = 0;
+          const int sx = cell_image(g, 0, cx0 + lane, kx);
+          ckey[ncl + lane] = sx >= 0 ? cell_key(g, sx, sy, sz) : -1;
+          ccode[ncl + lane] = (int)list_code(kx, ky, kz);
         }
-        const int pb = a.cstart[rowbase + s0], pe = a.cstart[rowbase + s1c + 1];
-        for (int base = pb; base < pe; base += 64) {
-          const int m = min(64, pe - base);
-          if (lane < m) {
-            sx[lane] = a.P4[base + lane];
-            se[lane] = ((unsigned)(base + lane) << 7) | code;
-          }
-          __syncthreads();
-          if (act) {
-            unsigned long long mask = 0ull;
-            for (int t = 0; t < m; t++) {
-              const double4 c = sx[t];
-              const float dx = (float)(xs[0] - c.x);
-              const float dy = NDIM >= 2 ? (float)(xs[1] - c.y) : 0.f;
-              const float dz = NDIM >= 3 ? (float)(xs[2] - c.z) : 0.f;
-              const float r2f = dx * dx + dy * dy + dz * dz;
-              if (r2f * hi2f < lim2f) mask |= 1ull << t;
-            }
-            while (mask) {
-              const int t = __builtin_ctzll(mask);
-              mask &= mask - 1ull;
-              const double4 c = sx[t];
-              const double xr[3] = {c.x, c.y, c.z};
-              const double rr = min_image_r2<NDIM>(g, xi, xr);
-              if (!(rr * hi21 < lim2)) continue;
-              const unsigned e = se[t];
-              if (cnt < a.maxnb) L[cnt] = e;
-              else over = true;
-              cnt++;
-              const int kk = (int)(e >> 7);
-              // same operands as kk's own test of k (rr is symmetric)
-              if (kk != k && !(rr * (c.w * c.w) < lim2)) {
-                const int slot = atomicAdd(&a.nbx[kk], 1);
-                if (slot < a.maxnb)
-                  a.nbl[(size_t)kk * a.maxnb + (a.maxnb - 1 - slot)] = ((unsigned)k << 7) | inverse_code(code);
-              }
-            }
-          }
-          __syncthreads();
-        }
+        ncl += nadd;
       }
     }
   }
+  if (ncl) process_batch();
   if (act) {
     a.nbc[k] = over ? -1 : cnt;
     a.hcap[k] = s1 / pk.w;
@@ -579,7 +737,7 @@ __global__ __launch_bounds__(64) void k_nblist(KArgs a, Phys p, Grid g) {
 // the gather list and the extras falls back to the cell scan
 __global__ __launch_bounds__(BS) void k_nblist_sort(KArgs a) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= a.n) return;
+  if (k >= a.n || !a.sc->list_on) return;
   const int c0 = a.nbc[k], nx = a.nbx[k];
   if (c0 < 0 || c0 + nx > a.maxnb) {
     if (c0 >= 0) a.nbc[k] = -1;
@@ -636,7 +794,8 @@ __device__ __forceinline__ void block_partials(double (&v)[NQ], const int (&kind
 }
 
 __global__ __launch_bounds__(BS) void k_red_final(const double* __restrict__ red, int rcap, int nblocks,
-                                                  RedSpec spec, DScal* sc) {
+                                                  RedSpec spec, DScal* sc, const int* skip) {
+  if (skip && *skip) return;
   __shared__ double sm[BS];
   for (int q = 0; q < spec.nq; q++) {
     const int kind = spec.op[q] & 15, type = spec.op[q] >> 4;
@@ -731,8 +890,17 @@ __device__ __forceinline__ void nr_update(const Phys& p, double hh, double hin, 
 }
 
 template <int NDIM, typename PT, bool DUST>
-__global__ __launch_bounds__(BS) void k_nr_pass(KArgs a, Phys p, Grid g, const int* __restrict__ list, int nlist,
-                                                int its0, int sym, int persistent, int* __restrict__ next_list) {
+__global__ __launch_bounds__(BS) void k_nr_pass(KArgs a, Phys p, const Grid* __restrict__ gp, int* __restrict__ list_a,
+                                                int* __restrict__ list_b) {
+  // pass parameters from the device-side control (k_pass_ctl): pass its0
+  // reads the active list written by the previous pass (list_a after odd
+  // passes, list_b after even ones) and appends its unconverged particles to
+  // the other one
+  if (a.sc->pc_done) return;
+  const int nlist = a.sc->pc_nlist, its0 = a.sc->pc_its, sym = a.sc->pc_sym;
+  const int persistent = a.sc->pc_persistent;
+  const int* __restrict__ list = its0 == 1 ? nullptr : ((its0 & 1) ? list_b : list_a);
+  int* __restrict__ next_list = (its0 & 1) ? list_a : list_b;
   __shared__ PT tw[NDSP_IKERN + 1];
   __shared__ PT tg[NDSP_IKERN + 1];
   __shared__ PT tgg[NDSP_IKERN + 1];
@@ -745,7 +913,8 @@ __global__ __launch_bounds__(BS) void k_nr_pass(KArgs a, Phys p, Grid g, const i
     }
   }
   __syncthreads();
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const Grid g = *gp;
+  const int tid = xcd_tile() * blockDim.x + threadIdx.x;
   const bool active = tid < nlist;
   bool unconverged = false;
   const int k = active ? (list ? list[tid] : tid) : 0;
@@ -769,6 +938,18 @@ __global__ __launch_bounds__(BS) void k_nr_pass(KArgs a, Phys p, Grid g, const i
     const double R2 = p.radkern2;
     const double hmaxc = a.sc->hmax_cur;
     constexpr bool dust = DUST;
+    struct DRec {
+      double4 p;
+      T4 d;
+      PT e;
+    };
+    auto loadj = [&](int kk) {
+      DRec r;
+      r.p = a.P4[kk];
+      r.d = D4[kk];
+      r.e = dust ? De[kk] : PT(0);
+      return r;
+    };
 
     int its = its0 - 1, nn = 0;
     unsigned long long npairs = 0;
@@ -795,19 +976,19 @@ __global__ __launch_bounds__(BS) void k_nr_pass(KArgs a, Phys p, Grid g, const i
       // symmetric count of the first pass (neighbours still at their build h)
       const bool uselist = p.use_list && a.nbc[k] >= 0 && h <= a.hcap[k] && (!symp || its == 1);
       const int nlisted = uselist ? (symp ? a.nbc[k] + a.nbx[k] : a.nbc[k]) : 0;
-      auto body = [&](int kk, const double* xj, const int* flip, int code) {
+      auto body = [&](int kk, const DRec& rj, const double* xj, const int* flip, int code) {
         double dx[3] = {0.0, 0.0, 0.0};
         for (int d = 0; d < NDIM; d++) dx[d] = xi[d] - xj[d];
         const double rij2 = rij2_exact<NDIM>(dx);
         const double q2i = mul_exact(rij2, hi21);
         if (symp) {
-          const double hj1 = a.P4[kk].w;
+          const double hj1 = rj.p.w;
           const double q2j = mul_exact(mul_exact(rij2, hj1), hj1);
           if (q2i < R2 || q2j < R2) nsym++;
         }
         if (!(q2i < R2)) return;
         ngat++;
-        const PT rij = sqrt((PT)rij2);
+        const PT rij = p_sqrt((PT)rij2);
         const PT q2 = (PT)q2i;
         const PT wabi = ktab(tw, q2, ddq2, dq2) * hfacwabi;
         const PT grkerni = ktab(tg, q2, ddq2, dq2) * hfacgrkerni;
@@ -815,26 +996,26 @@ __global__ __launch_bounds__(BS) void k_nr_pass(KArgs a, Phys p, Grid g, const i
         const PT dwdhi = -(rij * grkerni * hi1p) - PT(NDIM) * wabi * hi1p;
         const PT dwdhdhi = PT(NDIM * (NDIM + 1)) * wabi * hi1s + PT(2 * (NDIM + 1)) * rij * hi1s * grkerni +
                            rij * rij * hi1s * grgrkerni;
-        const T4 dj = D4[kk];
+        const T4 dj = rj.d;
         const PT mj = dj.w;
         rho.add(mj * wabi);
         gradh.add(mj * dwdhi);
         ggh.add(mj * dwdhdhi);
         if (dust) {
-          const PT epsj = De[kk];
+          const PT epsj = rj.e;
           rhod.add(mj * epsj * wabi);
           rhog.add(mj * (PT(1) - epsj) * wabi);
         }
         if (kk != k || code != 0) {
-          const PT rinv = PT(1) / (rij + (PT)DBL_EPSILON);
+          const PT rinv = p_rcp(rij + (PT)DBL_EPSILON);
           const PT vj[3] = {flip[0] ? -dj.x : dj.x, flip[1] ? -dj.y : dj.y, flip[2] ? -dj.z : dj.z};
           PT dvdotr = PT(0);
           for (int d = 0; d < NDIM; d++) dvdotr += (vi[d] - vj[d]) * ((PT)dx[d] * rinv);
           drhodt.add(mj * dvdotr * grkerni);
         }
       };
-      if (uselist) for_each_listed<NDIM>(a, g, k, nlisted, body);
-      else for_each_candidate<NDIM>(a, g, xi, rad, body);
+      if (uselist) for_each_listed_pf<NDIM>(a, g, k, nlisted, loadj, body);
+      else for_each_candidate_rec<NDIM>(a, g, xi, rad, loadj, body);
       npairs += (unsigned long long)ngat;
       nn = symp ? nsym : ngat;
       // Newton-Raphson update, src/iterate_density.f90:162-278 (FP64)
@@ -937,8 +1118,11 @@ __global__ __launch_bounds__(BS) void k_nr_pass(KArgs a, Phys p, Grid g, const i
   }
 }
 
-// P4.w = 1/h of every particle (before a symmetric pass after the first)
-__global__ void k_refresh_hinv(KArgs a, Phys p, int refresh_dxb) {
+// P4.w = 1/h of every particle before a symmetric pass after the first, and
+// the reflecting ghost extents after a relink (as decided by k_pass_ctl)
+__global__ void k_refresh_hinv(KArgs a, Phys p) {
+  if (a.sc->pc_done || !a.sc->pc_refresh) return;
+  const int refresh_dxb = a.sc->pc_relink;
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= a.n) return;
   const double h = a.f[F_H][a.sidx[k]];
@@ -947,7 +1131,9 @@ __global__ void k_refresh_hinv(KArgs a, Phys p, int refresh_dxb) {
 }
 
 // max h over all particles -> hmax_cur (and the ghost extent on a relink)
-__global__ void k_hmax(KArgs a, DScal* sc, int relink) {
+__global__ void k_hmax(KArgs a, DScal* sc) {
+  if (sc->pc_done) return;
+  const int relink = sc->pc_relink;
   __shared__ double sm[BS];
   double hm = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x)
@@ -964,13 +1150,32 @@ __global__ void k_hmax(KArgs a, DScal* sc, int relink) {
   }
 }
 
-__global__ void k_next_pass_scalars(DScal* sc, int relink) {
+// After a pass (src/iterate_density.f90:280-300): finished when the pass was
+// the persistent one, failed, or left no particle unconverged; otherwise the
+// unconverged ones form the next pass, symmetric if they are all particles
+// (:131-135), synchronous while walls reflect (a relink, :260-262, rebuilds
+// the ghosts with every particle's current h).
+__global__ void k_pass_ctl(DScal* sc, int n, int reflect, int maxits) {
   if (threadIdx.x | blockIdx.x) return;
+  if (sc->pc_done) return;
+  if (sc->pc_persistent || sc->err || sc->ncalc_next == 0 || sc->pc_its > maxits) {
+    sc->pc_done = 1;
+    return;
+  }
+  const int relink = reflect && sc->relink;
+  const int nl = sc->ncalc_next;
+  sc->pc_nlist = nl;
+  sc->pc_its = sc->pc_its + 1;
+  sc->pc_sym = nl == n;
+  sc->pc_persistent = !reflect && nl != n;
+  sc->pc_relink = relink;
+  sc->pc_refresh = nl == n || relink;
   sc->ncalc_next = 0;
   sc->relink = 0;
   sc->hmax_cur = 0.0;
   if (relink) sc->hhmax_ghost = 0.0;
 }
+
 
 // ===========================================================================
 // force: fused pair sums of get_rates
@@ -978,14 +1183,15 @@ __global__ void k_next_pass_scalars(DScal* sc, int relink) {
 
 // PHYS: 0 hydro, 1 MHD (imhd 11), 2 one-fluid dust (idust 4)
 template <int NDIM, typename PT, int PHYS>
-__global__ __launch_bounds__(BS) void k_force(KArgs a, Phys p, Grid g) {
+__global__ __launch_bounds__(BS) void k_force(KArgs a, Phys p, const Grid* __restrict__ gp) {
   __shared__ PT tg[NDSP_IKERN + 1];
   {
     const PT* src = sizeof(PT) == 4 ? (const PT*)a.tabf : (const PT*)a.tabd;
     for (int t = threadIdx.x; t <= NDSP_IKERN; t += blockDim.x) tg[t] = src[NDSP_IKERN + 1 + t];
   }
   __syncthreads();
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const Grid g = *gp;
+  const int k = xcd_tile() * blockDim.x + threadIdx.x;
   const bool active = k < a.n;
   typedef typename V4<PT>::type T4;
   const T4* G0 = (const T4*)a.G0;
@@ -1013,7 +1219,7 @@ __global__ __launch_bounds__(BS) void k_force(KArgs a, Phys p, Grid g) {
     const T4 g0 = G0[k], g1 = G1[k], g2 = G2[k], g4 = G4[k];
     const PT vi[3] = {g0.x, g0.y, g0.z};
     const PT mi = g0.w;
-    const PT rhoi = g1.x, rho1i = PT(1) / rhoi, rho21i = rho1i * rho1i;
+    const PT rhoi = g1.x, rho1i = p_rcp(rhoi), rho21i = rho1i * rho1i;
     const PT pri = fmax(g1.y, PT(0));
     const PT Prho2i = pri * rho21i;
     const PT csi = g1.z, gradhi = g1.w;
@@ -1041,22 +1247,39 @@ __global__ __launch_bounds__(BS) void k_force(KArgs a, Phys p, Grid g) {
     del2u.zero(); deps.zero();
 
     const bool uselist = p.use_list && a.nbc[k] >= 0 && !a.sc->list_invalid;
-    auto body = [&](int kk, const double* xj, const int* flip, int code) {
+    struct FRec {
+      double4 p;
+      T4 g0, g1, g2, g3, g4;
+      PT g5;
+    };
+    auto loadj = [&](int kk) {
+      FRec r;
+      r.p = a.P4[kk];
+      r.g0 = G0[kk];
+      r.g1 = G1[kk];
+      r.g2 = G2[kk];
+      r.g4 = G4[kk];
+      if (mhd) r.g3 = G3[kk];
+      else r.g3 = T4{PT(0), PT(0), PT(0), PT(0)};
+      r.g5 = dust ? G5[kk] : PT(0);
+      return r;
+    };
+    auto body = [&](int kk, const FRec& rj, const double* xj, const int* flip, int code) {
       double dx[3] = {0.0, 0.0, 0.0};
       for (int d = 0; d < NDIM; d++) dx[d] = xi[d] - xj[d];
       const double rij2 = rij2_exact<NDIM>(dx);
       const double q2i = mul_exact(rij2, hi21);
-      const double hj1 = a.P4[kk].w;
+      const double hj1 = rj.p.w;
       const double q2j = mul_exact(mul_exact(rij2, hj1), hj1);
       if (!(q2i < R2 || q2j < R2)) return;
       if (kk == k && code == 0) return;
       npairs++;
       if (rij2 <= 0.0) { err = NDSP_ERR_COINCIDENT; return; }
-      const PT rij = sqrt((PT)rij2);
-      const PT rinv = PT(1) / rij;
+      const PT rij = p_sqrt((PT)rij2);
+      const PT rinv = p_rcp(rij);
       PT dr[3] = {0, 0, 0};
       for (int d = 0; d < NDIM; d++) dr[d] = (PT)dx[d] * rinv;
-      const T4 j0 = G0[kk], j1 = G1[kk], j2 = G2[kk], j4 = G4[kk];
+      const T4 j0 = rj.g0, j1 = rj.g1, j2 = rj.g2, j4 = rj.g4;
       const PT hj1p = (PT)hj1;
       const PT hfacgrkernj = pown_t<NDIM>(hj1p) * hj1p;
       const PT grkerni = ktab(tg, (PT)q2i, ddq2, dq2) * hfacgrkerni * gradhi;
@@ -1066,7 +1289,7 @@ __global__ __launch_bounds__(BS) void k_force(KArgs a, Phys p, Grid g) {
       const PT mj = j0.w;
       const PT dvel[3] = {vi[0] - vj[0], vi[1] - vj[1], vi[2] - vj[2]};
       const PT dvdotr = dvel[0] * dr[0] + dvel[1] * dr[1] + dvel[2] * dr[2];
-      const PT rhoj = j1.x, rho1j = PT(1) / rhoj, rho21j = rho1j * rho1j;
+      const PT rhoj = j1.x, rho1j = p_rcp(rhoj), rho21j = rho1j * rho1j;
       const PT rhoav1 = PT(0.5) * (rho1i + rho1j);
       const PT prj = fmax(j1.y, PT(0));
       const PT Prho2j = prj * rho21j;
@@ -1076,7 +1299,7 @@ __global__ __launch_bounds__(BS) void k_force(KArgs a, Phys p, Grid g) {
       PT projBi = 0, projBj = 0, projdB = 0, projBrhoi = 0, projBrhoj = 0, Brho2j = 0, valfven2j = 0;
       PT Brhoj[3] = {0, 0, 0};
       if (mhd) {
-        const T4 j3 = G3[kk];
+        const T4 j3 = rj.g3;
         Bj[0] = j3.x; Bj[1] = j3.y; Bj[2] = j3.z; psij = j3.w;
         for (int c = 0; c < 3; c++) { Brhoj[c] = Bj[c] * rho1j; dB[c] = Bi[c] - Bj[c]; }
         projBi = Bi[0] * dr[0] + Bi[1] * dr[1] + Bi[2] * dr[2];
@@ -1085,7 +1308,7 @@ __global__ __launch_bounds__(BS) void k_force(KArgs a, Phys p, Grid g) {
         projBrhoi = Brhoi[0] * dr[0] + Brhoi[1] * dr[1] + Brhoi[2] * dr[2];
         projBrhoj = Brhoj[0] * dr[0] + Brhoj[1] * dr[1] + Brhoj[2] * dr[2];
         const PT B2j = Bj[0] * Bj[0] + Bj[1] * Bj[1] + Bj[2] * Bj[2];
-        valfven2j = B2j / j4.y;
+        valfven2j = p_div(B2j, j4.y);
         Brho2j = B2j * rho21j;
       }
       // signal velocities, src/ratesND_mhd.f90:1417-1482
@@ -1096,11 +1319,11 @@ __global__ __launch_bounds__(BS) void k_force(KArgs a, Phys p, Grid g) {
         const PT ti = csi * projBi, tj = csj * projBj;
         const PT vsigproji = vsig2i * vsig2i - PT(4) * (ti * ti) * rho1i;
         const PT vsigprojj = vsig2j * vsig2j - PT(4) * (tj * tj) * rho1j;
-        vsigi = sqrt(PT(0.5) * (vsig2i + sqrt(fmax(vsigproji, PT(0)))));
-        vsigj = sqrt(PT(0.5) * (vsig2j + sqrt(fmax(vsigprojj, PT(0)))));
+        vsigi = p_sqrt(PT(0.5) * (vsig2i + p_sqrt(fmax(vsigproji, PT(0)))));
+        vsigj = p_sqrt(PT(0.5) * (vsig2j + p_sqrt(fmax(vsigprojj, PT(0)))));
         if (vsigproji < PT(-1.e-4) * vsig2i * vsig2i || vsigprojj < PT(-1.e-4) * vsig2j * vsig2j)
           err = NDSP_ERR_VSIG;
-        if (p.iavlim[2] != 2) vsigB = sqrt(dvel[0] * dvel[0] + dvel[1] * dvel[1] + dvel[2] * dvel[2]);
+        if (p.iavlim[2] != 2) vsigB = p_sqrt(dvel[0] * dvel[0] + dvel[1] * dvel[1] + dvel[2] * dvel[2]);
         else vsigB = PT(0.5) * (vsigi + vsigj) + fabs(dvdotr);
       } else {
         vsigi = csi;
@@ -1108,10 +1331,10 @@ __global__ __launch_bounds__(BS) void k_force(KArgs a, Phys p, Grid g) {
         vsigB = PT(0);
       }
       const PT vsig = PT(0.5) * fmax(vsigi + vsigj - beta * dvdotr, PT(0));
-      const PT vsigu = sqrt(fabs(pri - prj) * rhoav1);
+      const PT vsigu = p_sqrt(fabs(pri - prj) * rhoav1);
       const PT vsigdtc = fmax(fmax(vsig, PT(0.5) * (vsigi + vsigj + beta * fabs(dvdotr))), vsigB);
       vsigmax_l = fmax(vsigmax_l, vsigdtc);
-      if (vsigdtc > PT(1.e-10)) dtc_l = fmin(dtc_l, (double)(fmin(hi, hj) / vsigdtc));
+      if (vsigdtc > PT(1.e-10)) dtc_l = fmin(dtc_l, (double)p_div((PT)fmin(hi, hj), vsigdtc));
       PT f[3] = {0, 0, 0};
       PT du = PT(0);
       // artificial dissipation
@@ -1128,7 +1351,7 @@ __global__ __launch_bounds__(BS) void k_force(KArgs a, Phys p, Grid g) {
           const PT cfaci = PT(0.5) * alphaui * rhoi * vsigu * duu;
           const PT cfacj = PT(0.5) * j2.z * rhoj * vsigu * duu;
           const PT diffu = cfaci * grkerni * rho21i + cfacj * grkernj * rho21j;
-          du += PT(0.5) * qi * rho1i / rhogasi * mj * dvdotr * grkerni + mj * diffu / (PT(1) - epsi);
+          du += PT(0.5) * qi * p_div(rho1i, rhogasi) * mj * dvdotr * grkerni + p_div(mj * diffu, PT(1) - epsi);
         }
       } else {
         // src/ratesND_mhd.f90:1700-1894
@@ -1191,14 +1414,14 @@ __global__ __launch_bounds__(BS) void k_force(KArgs a, Phys p, Grid g) {
       if (dust) {
         // dust_derivs_diffusion, idustevol 0, src/ratesND_mhd.f90:2813-2878
         const PT Di = rhodusti * rho1i * tstop;
-        const PT Dj = G5[kk] * rho1j * tstop;
+        const PT Dj = rj.g5 * rho1j * tstop;
         const PT Dav = PT(0.5) * (Di + Dj);
         const PT grgrkern = PT(-2) * grkern * rinv;
         const PT diffterm = rho1i * rho1j * Dav * (pri - prj) * grgrkern;
         deps.add(mj * diffterm);
         if (p.iener > 0) {
           const PT duu = uui - j2.x;
-          du += pri * rho1i / rhogasi * mj * dvdotr * grkerni - PT(0.5) * mj * duu * diffterm / (PT(1) - epsi);
+          du += pri * p_div(rho1i, rhogasi) * mj * dvdotr * grkerni - p_div(PT(0.5) * mj * duu * diffterm, PT(1) - epsi);
         }
       }
       fx.add(f[0]);
@@ -1208,8 +1431,8 @@ __global__ __launch_bounds__(BS) void k_force(KArgs a, Phys p, Grid g) {
       if (p.iavlim[1] > 0) del2u.add(mj * rho1j * ((uui - j2.x) * rinv) * grkerni);
     };
     if (!err) {
-      if (uselist) for_each_listed<NDIM>(a, g, k, a.nbc[k] + a.nbx[k], body);
-      else for_each_candidate<NDIM>(a, g, xi, rad, body);
+      if (uselist) for_each_listed_pf<NDIM>(a, g, k, a.nbc[k] + a.nbx[k], loadj, body);
+      else for_each_candidate_rec<NDIM>(a, g, xi, rad, loadj, body);
     }
     a.f[F_FX][i] = fx.value();
     a.f[F_FY][i] = fy.value();
@@ -1333,8 +1556,23 @@ __global__ void k_dt(DScal* sc, Phys p) {
                 p.C_force * DBL_MAX);
 }
 
-__global__ void k_init_density_scalars(DScal* sc, double bconstmax) {
+// end of a leapfrog step: advance the time by the step's dt and take the dt
+// chosen by this step's rates for the next one (src/evolve.f90:106-110)
+__global__ void k_step_end(DScal* sc) {
   if (threadIdx.x | blockIdx.x) return;
+  sc->time = sc->time + sc->dt_cur;
+  sc->dt_cur = sc->dt;
+}
+
+__global__ void k_init_density_scalars(DScal* sc, double bconstmax, int n) {
+  if (threadIdx.x | blockIdx.x) return;
+  sc->pc_nlist = n;
+  sc->pc_its = 1;
+  sc->pc_sym = 1;
+  sc->pc_persistent = 0;
+  sc->pc_done = 0;
+  sc->pc_relink = 0;
+  sc->pc_refresh = 0;
   sc->hmax_nr = 0.0;
   sc->hmax_cur = sc->hmax_bin;
   sc->hhmax_ghost = sc->hmax_bin;
@@ -1362,10 +1600,11 @@ __global__ void k_init_force_scalars(DScal* sc) {
 // leapfrog, src/stepND_leapfrog_mhd.f90:31-275
 // ===========================================================================
 
-__global__ void k_predict(KArgs a, Phys p, double dt) {
+__global__ void k_predict(KArgs a, Phys p) {
 #pragma clang fp contract(off)
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
+  const double dt = a.sc->dt_cur;
   double** f = a.f;
   static const int xs[3] = {F_X, F_Y, F_Z}, vs[3] = {F_VX, F_VY, F_VZ}, bs[3] = {F_BX, F_BY, F_BZ};
   static const int as[3] = {F_A0, F_A1, F_A2}, fs[3] = {F_FX, F_FY, F_FZ}, dbs[3] = {F_DBX, F_DBY, F_DBZ};
@@ -1400,10 +1639,11 @@ __global__ void k_predict(KArgs a, Phys p, double dt) {
   if (p.idust) f[F_EPS][i] = f[F_EPSIN][i] + dt * f[F_DEPSIN][i];
 }
 
-__global__ void k_correct(KArgs a, Phys p, double dt) {
+__global__ void k_correct(KArgs a, Phys p) {
 #pragma clang fp contract(off)
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
+  const double dt = a.sc->dt_cur;
   double** f = a.f;
   const double hdt = 0.5 * dt;
   static const int vs[3] = {F_VX, F_VY, F_VZ}, bs[3] = {F_BX, F_BY, F_BZ}, as[3] = {F_A0, F_A1, F_A2};
@@ -1515,6 +1755,14 @@ __global__ void k_gather_d(const double* __restrict__ in, double* __restrict__ o
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) out[k] = in[sidx[k]];
 }
+__global__ void k_copy_d(const double* __restrict__ in, double* __restrict__ out, int n) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = in[k];
+}
+__global__ void k_copy_i(const int* __restrict__ in, int* __restrict__ out, int n) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = in[k];
+}
 __global__ void k_gather_i(const int* __restrict__ in, int* __restrict__ out, const int* sidx, int n) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) out[k] = in[sidx[k]];
@@ -1525,9 +1773,10 @@ __global__ void k_gather_i(const int* __restrict__ in, int* __restrict__ out, co
 // ===========================================================================
 
 template <int NDIM>
-__global__ void k_nb(KArgs a, Phys p, Grid g, int sym, const int* offs, int* out) {
+__global__ void k_nb(KArgs a, Phys p, const Grid* __restrict__ gp, int sym, const int* offs, int* out) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= a.n) return;
+  const Grid g = *gp;
   const double4 pk = a.P4[k];
   const double xi[3] = {pk.x, pk.y, pk.z};
   const double hi21 = mul_exact(pk.w, pk.w);
@@ -1664,7 +1913,13 @@ static double nexp_of(const Phys& p) {
 }
 
 struct ndsp_ctx {
-  bool list_on = false;  // neighbour lists built by the last binning
+  bool reflect = false;     // some wall reflects
+  Grid* gdev = nullptr;     // cell grid of the last binning (device)
+  int cap_bits = 0;         // Morton key-space capacity of cstart/cfill
+  bool capturing = false;   // a step is being recorded into a graph
+  bool use_graph = false;   // ndsp_step replays recorded graphs
+  bool graph_valid = false;
+  hipGraphExec_t graph_a = nullptr, graph_b = nullptr;
   Phys p;
   int prec;
   int n;
@@ -1681,7 +1936,6 @@ struct ndsp_ctx {
   double* part = nullptr;       // reduction partials
   double* cons_out = nullptr;
   int nb_part = 0;
-  long long ncell_cap = 0;
   Grid g;
   bool grid_valid = false;
   bool have_rates = false;
@@ -1714,7 +1968,7 @@ struct TScope {
   ndsp_ctx* c;
   int lab, ev;
   TScope(ndsp_ctx* c_, const char* name) : c(c_), lab(-1), ev(-1) {
-    if (!c->timing) return;
+    if (!c->timing || c->capturing) return;
     lab = label_id(c, name);
     if (c->nev >= (int)c->ev0.size()) {
       hipEvent_t a0, a1;
@@ -1772,10 +2026,42 @@ static int check_err(ndsp_ctx* c, const DScal& h) {
 }
 
 // set_linklist: boundary, statistics, grid, counting sort, density view
+// Morton key-space capacity (bits) for a grid of the current h: 2^ndim times
+// the key space of the present grid, at most 2^28 keys
+static int size_key_space(ndsp_ctx* c, const DScal& hs, int extra_bits) {
+  const double rmax = c->p.radkern * hs.hmax_bin;
+  if (!(rmax > 0.0) || !std::isfinite(rmax)) return set_err(NDSP_ERR_CELLS, "link: max h <= 0");
+  const double cs = rmax / NDSP_CELL_SUB;
+  int mx = 1;
+  for (int d = 0; d < c->p.ndim; d++) {
+    const double ext = c->ibound[d] >= 2 ? c->xmax[d] - c->xmin[d] : hs.bbox_hi[d] - hs.bbox_lo[d];
+    const double ncd = std::floor(ext / cs) + 1.0;
+    mx = std::max(mx, (int)std::min(ncd, 1.0e8));
+  }
+  int mb = 0;
+  while ((1 << mb) < mx) mb++;
+  const int bits = std::min(28, c->p.ndim * (mb + extra_bits));
+  if (bits == c->cap_bits) return 0;
+  const long long cap = (1LL << bits) + 1;
+  if (c->a.cstart) hipFree(c->a.cstart);
+  if (c->a.cfill) hipFree(c->a.cfill);
+  if (c->bsum) hipFree(c->bsum);
+  c->a.cstart = c->a.cfill = c->bsum = nullptr;
+  HIPCHK(hipMalloc(&c->a.cstart, sizeof(int) * cap));
+  HIPCHK(hipMalloc(&c->a.cfill, sizeof(int) * cap));
+  HIPCHK(hipMalloc(&c->bsum, sizeof(int) * (cap / (BS * 4) + 2)));
+  c->cap_bits = bits;
+  c->graph_valid = false;
+  return 0;
+}
+
+// Binning (set_linklist, src/linkND.f90:45): wrap, statistics, device-side
+// grid, counting sort by Morton cell key, packed views, neighbour lists.
+// No host round trip except the first call, which sizes the key space.
 static int do_linklist(ndsp_ctx* c, int in_step) {
   KArgs& a = c->a;
   const int n = c->n;
-  HIPCHK(hipMemsetAsync(&a.sc->err, 0, sizeof(int), c->st));
+  hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, c->st, &a.sc->err, 1, 0);
   {
     TScope t(c, "boundary");
     hipLaunchKernelGGL(k_boundary, dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p, c->g, in_step);
@@ -1785,65 +2071,30 @@ static int do_linklist(ndsp_ctx* c, int in_step) {
     hipLaunchKernelGGL(k_binstats, dim3(c->nb_part), dim3(BS), 0, c->st, a, c->p.ndim, c->part);
     hipLaunchKernelGGL(k_binstats_final, dim3(1), dim3(BS), 0, c->st, a.sc, c->part, c->nb_part);
   }
-  DScal hs;
-  if (int e = sync_scalars(c, hs)) return e;
-  const double hmax = hs.hmax_bin;
-  const double rmax = c->p.radkern * hmax;
-  if (!(rmax > 0.0) || !std::isfinite(rmax)) return set_err(NDSP_ERR_CELLS, "link: max h <= 0");
-  // cells of side >= radkern*hmax/2: a search visits fewer candidates than
-  // with whole-radius cells once rows are cut to the sphere
-  const double cs = rmax / NDSP_CELL_SUB;
-  Grid g = c->g;
-  long long ncell = 1;
-  for (int d = 0; d < 3; d++) {
-    if (d >= c->p.ndim) {
-      g.nc[d] = 1; g.lo[d] = 0.0; g.w[d] = 1.0; g.inv_w[d] = 1.0;
-      continue;
-    }
-    if (g.bound[d] >= 2) {
-      const double L = g.xmax[d] - g.xmin[d];
-      if (L < rmax) return set_err(NDSP_ERR_CELLS, "link: radkern*hmax exceeds the box");
-      double ncd = std::floor(L / cs);
-      if (ncd > 1.0e8) return set_err(NDSP_ERR_CELLS, "link: too many cells");
-      g.nc[d] = std::max(1, (int)ncd);
-      g.lo[d] = g.xmin[d];
-      g.w[d] = L / g.nc[d];
-    } else {
-      const double ext = hs.bbox_hi[d] - hs.bbox_lo[d];
-      double ncd = std::floor(ext / cs) + 1.0;
-      if (ncd > 1.0e8) return set_err(NDSP_ERR_CELLS, "link: too many cells");
-      g.nc[d] = (int)ncd;
-      g.lo[d] = hs.bbox_lo[d];
-      g.w[d] = cs;
-    }
-    g.inv_w[d] = 1.0 / g.w[d];
-    ncell *= g.nc[d];
+  if (c->cap_bits == 0) {
+    if (c->capturing) return set_err(NDSP_ERR_BAD_ARG, "capture before the first binning");
+    DScal hs;
+    if (int e = sync_scalars(c, hs)) return e;
+    if (int e = size_key_space(c, hs, 1)) return e;
   }
-  if (ncell > (1LL << 28)) return set_err(NDSP_ERR_CELLS, "link: too many cells");
-  g.ncell = ncell;
-  if (ncell + 1 > c->ncell_cap) {
-    if (a.cstart) hipFree(a.cstart);
-    if (a.cfill) hipFree(a.cfill);
-    if (c->bsum) hipFree(c->bsum);
-    long long cap = (ncell + 1) * 5 / 4 + 1024;
-    HIPCHK(hipMalloc(&a.cstart, sizeof(int) * cap));
-    HIPCHK(hipMalloc(&a.cfill, sizeof(int) * cap));
-    HIPCHK(hipMalloc(&c->bsum, sizeof(int) * (cap / (BS * 4) + 2)));
-    c->ncell_cap = cap;
-  }
-  c->g = g;
+  const long long cap = (1LL << c->cap_bits) + 1;
+  const unsigned cap_blocks = (unsigned)std::min<long long>(nblk(cap), 1 << 20);
   {
     TScope t(c, "cellsort");
-    HIPCHK(hipMemsetAsync(a.cfill, 0, sizeof(int) * (ncell + 1), c->st));
-    hipLaunchKernelGGL(k_cellid, dim3(nblk(n)), dim3(BS), 0, c->st, a, g);
-    const long long m = ncell + 1;
-    const int nbs = (int)((m + BS * 4 - 1) / (BS * 4));
-    hipLaunchKernelGGL(k_scan_local, dim3(nbs), dim3(BS), 0, c->st, (const int*)a.cfill, a.cstart, c->bsum, m);
-    hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(BS), 0, c->st, c->bsum, nbs, (int*)nullptr);
-    hipLaunchKernelGGL(k_scan_add, dim3(nblk(m)), dim3(BS), 0, c->st, a.cstart, (const int*)c->bsum, m);
-    HIPCHK(hipMemsetAsync(a.cfill, 0, sizeof(int) * (ncell + 1), c->st));
+    hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(64), 0, c->st, a.sc, c->gdev, c->g, c->p, c->cap_bits);
+    hipLaunchKernelGGL(k_fill_cells, dim3(std::min(cap_blocks, 4096u)), dim3(BS), 0, c->st, a.cfill,
+                       (const Grid*)c->gdev);
+    hipLaunchKernelGGL(k_cellid, dim3(nblk(n)), dim3(BS), 0, c->st, a, (const Grid*)c->gdev);
+    const unsigned nbs = (unsigned)((cap + BS * 4 - 1) / (BS * 4));
+    hipLaunchKernelGGL(k_scan_local, dim3(nbs), dim3(BS), 0, c->st, (const int*)a.cfill, a.cstart, c->bsum,
+                       (const Grid*)c->gdev);
+    hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(BS), 0, c->st, c->bsum, (const Grid*)c->gdev, (int*)nullptr);
+    hipLaunchKernelGGL(k_scan_add, dim3(cap_blocks), dim3(BS), 0, c->st, a.cstart, (const int*)c->bsum,
+                       (const Grid*)c->gdev);
+    hipLaunchKernelGGL(k_fill_cells, dim3(std::min(cap_blocks, 4096u)), dim3(BS), 0, c->st, a.cfill,
+                       (const Grid*)c->gdev);
     hipLaunchKernelGGL(k_scatter, dim3(nblk(n)), dim3(BS), 0, c->st, a);
-    hipLaunchKernelGGL(k_cellsort, dim3(nblk(ncell)), dim3(BS), 0, c->st, a, ncell);
+    hipLaunchKernelGGL(k_cellsort, dim3(cap_blocks), dim3(BS), 0, c->st, a, (const Grid*)c->gdev);
   }
   {
     TScope t(c, "pack_density");
@@ -1852,18 +2103,12 @@ static int do_linklist(ndsp_ctx* c, int in_step) {
     else
       hipLaunchKernelGGL(k_pack_density<float>, dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p);
   }
-  // lists need the minimum image to be unique: 2 radkern hmax (1+skin) < L
-  c->list_on = c->p.use_list != 0;
-  for (int d = 0; d < c->p.ndim; d++)
-    if (g.bound[d] == 3 && !(2.0 * rmax * (1.0 + c->p.skin) * 1.001 < g.xmax[d] - g.xmin[d]))
-      c->list_on = false;
-  if (c->list_on) {
+  if (c->p.use_list) {
     TScope t(c, "nblist");
-    HIPCHK(hipMemsetAsync(&a.sc->list_overflow, 0, sizeof(int), c->st));
-    HIPCHK(hipMemsetAsync(a.nbx, 0, sizeof(int) * n, c->st));
+    hipLaunchKernelGGL(k_fill_i, dim3(std::min(nblk(n), 4096u)), dim3(BS), 0, c->st, a.nbx, n, 0);
     dispatch_ndim(c->p.ndim, [&](auto nd) {
       constexpr int ND = decltype(nd)::value;
-      hipLaunchKernelGGL((k_nblist<ND>), dim3(nblk(n, 64)), dim3(64), 0, c->st, a, c->p, g);
+      hipLaunchKernelGGL((k_nblist<ND>), dim3(nblk(n, 64)), dim3(64), 0, c->st, a, c->p, (const Grid*)c->gdev);
       return 0;
     });
     hipLaunchKernelGGL(k_nblist_sort, dim3(nblk(n)), dim3(BS), 0, c->st, a);
@@ -1873,54 +2118,60 @@ static int do_linklist(ndsp_ctx* c, int in_step) {
   return 0;
 }
 
-static int do_density(ndsp_ctx* c) {
+// K pass launches of the h-rho iteration; each reads its parameters from the
+// device-side control and does nothing once the iteration has finished
+static void launch_passes(ndsp_ctx* c, int K) {
   KArgs& a = c->a;
   const int n = c->n;
-  hipLaunchKernelGGL(k_init_density_scalars, dim3(1), dim3(1), 0, c->st, a.sc, c->bconstmax);
-  bool reflect = false;
-  for (int d = 0; d < c->p.ndim; d++) if (c->ibound[d] == 2) reflect = true;
-  int its = 1, nlist = n;
-  const int* list = nullptr;
-  int* next = c->list_a;
-  int* other = c->list_b;
-  TScope t(c, "density");
-  while (true) {
-    const int sym = (nlist == n);
-    const int persistent = (!reflect && !sym) ? 1 : 0;
+  const bool reflect = c->reflect;
+  for (int q = 0; q < K; q++) {
     dispatch_ndim(c->p.ndim, [&](auto nd) {
       constexpr int ND = decltype(nd)::value;
       const bool fp64 = c->prec == NDSP_PREC_FP64, dust = c->p.idust != 0;
       auto kern = fp64 ? (dust ? k_nr_pass<ND, double, true> : k_nr_pass<ND, double, false>)
                        : (dust ? k_nr_pass<ND, float, true> : k_nr_pass<ND, float, false>);
-      Phys pl = c->p;
-      pl.use_list = c->list_on ? 1 : 0;
-      hipLaunchKernelGGL(kern, dim3(nblk(nlist)), dim3(BS), 0, c->st, a, pl, c->g, list, nlist, its, sym,
-                         persistent, next);
+      hipLaunchKernelGGL(kern, dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p, (const Grid*)c->gdev, c->list_a,
+                         c->list_b);
       return 0;
     });
-    hipLaunchKernelGGL(k_red_final, dim3(1), dim3(BS), 0, c->st, (const double*)a.red, a.rcap, (int)nblk(nlist),
+    hipLaunchKernelGGL(k_red_final, dim3(1), dim3(BS), 0, c->st, (const double*)a.red, a.rcap, (int)nblk(n),
                        red_spec({RED_Q(RED_MAX, RED_F64, stressmax), RED_Q(RED_MAX, RED_F64, hmax_nr),
                                  RED_Q(RED_MAX, RED_I32, itsmax), RED_Q(RED_MIN, RED_I32, nmin),
                                  RED_Q(RED_MAX, RED_I32, nmax), RED_Q(RED_SUM, RED_U64, pairs_density),
                                  RED_Q(RED_SUM, RED_U64, ncalc_total)}),
-                       a.sc);
-    HIPCHK(hipGetLastError());
-    if (persistent) break;
-    DScal hs;
-    if (int e = sync_scalars(c, hs)) return e;
-    if (hs.err || hs.ncalc_next == 0 || its > c->p.maxits) break;
-    const int relink = reflect && hs.relink;
-    nlist = hs.ncalc_next;
-    list = next;
-    std::swap(next, other);
-    its++;
-    hipLaunchKernelGGL(k_next_pass_scalars, dim3(1), dim3(1), 0, c->st, a.sc, relink);
-    hipLaunchKernelGGL(k_hmax, dim3(std::min(1024u, nblk(n))), dim3(BS), 0, c->st, a, a.sc, relink);
-    if (nlist == n || relink)
-      hipLaunchKernelGGL(k_refresh_hinv, dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p, relink);
+                       a.sc, (const int*)&a.sc->pc_done);
+    hipLaunchKernelGGL(k_pass_ctl, dim3(1), dim3(64), 0, c->st, a.sc, n, reflect ? 1 : 0, c->p.maxits);
+    hipLaunchKernelGGL(k_hmax, dim3(std::min(1024u, nblk(n))), dim3(BS), 0, c->st, a, a.sc);
+    hipLaunchKernelGGL(k_refresh_hinv, dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p);
   }
+}
+
+// Passes launched without a host decision in between: synchronous symmetric
+// passes while every particle iterates, then (no reflecting walls) one
+// persistent pass for the rest.  A recorded step holds the first chunk only;
+// density_finish completes the iteration after it when needed.
+static int pass_chunk(const ndsp_ctx* c) { return c->reflect ? 4 : 3; }
+
+static int do_density(ndsp_ctx* c) {
+  KArgs& a = c->a;
+  hipLaunchKernelGGL(k_init_density_scalars, dim3(1), dim3(1), 0, c->st, a.sc, c->bconstmax, c->n);
+  TScope t(c, "density");
+  launch_passes(c, pass_chunk(c));
   HIPCHK(hipGetLastError());
   return 0;
+}
+
+// until the device control reports the iteration finished: more passes
+static int density_finish(ndsp_ctx* c) {
+  for (int guard = 0; guard < 1000; guard++) {
+    DScal hs;
+    if (int e = sync_scalars(c, hs)) return e;
+    if (hs.pc_done) return 0;
+    TScope t(c, "density");
+    launch_passes(c, pass_chunk(c));
+    HIPCHK(hipGetLastError());
+  }
+  return set_err(NDSP_ERR_NOT_CONVERGED, "h-rho iteration did not finish");
 }
 
 static int do_rates(ndsp_ctx* c) {
@@ -1939,26 +2190,25 @@ static int do_rates(ndsp_ctx* c) {
     dispatch_ndim(c->p.ndim, [&](auto nd) {
       constexpr int ND = decltype(nd)::value;
       const int phys = c->p.imhd ? 1 : c->p.idust ? 2 : 0;
-      void (*kern)(KArgs, Phys, Grid);
+      void (*kern)(KArgs, Phys, const Grid* __restrict__);
       if (c->prec == NDSP_PREC_FP64)
         kern = phys == 1 ? k_force<ND, double, 1> : phys == 2 ? k_force<ND, double, 2> : k_force<ND, double, 0>;
       else
         kern = phys == 1 ? k_force<ND, float, 1> : phys == 2 ? k_force<ND, float, 2> : k_force<ND, float, 0>;
-      Phys pl = c->p;
-      pl.use_list = c->list_on ? 1 : 0;
-      hipLaunchKernelGGL(kern, dim3(nblk(n)), dim3(BS), 0, c->st, a, pl, c->g);
+      hipLaunchKernelGGL(kern, dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p, (const Grid*)c->gdev);
       return 0;
     });
     hipLaunchKernelGGL(k_red_final, dim3(1), dim3(BS), 0, c->st, (const double*)a.red, a.rcap, (int)nblk(n),
                        red_spec({RED_Q(RED_MAX, RED_F64, vsigmax), RED_Q(RED_MIN, RED_F64, dtcourant),
                                  RED_Q(RED_SUM, RED_U64, pairs_force)}),
-                       a.sc);
+                       a.sc, (const int*)nullptr);
   }
   {
     TScope t(c, "finalize");
     hipLaunchKernelGGL(k_finalize, dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p);
     hipLaunchKernelGGL(k_red_final, dim3(1), dim3(BS), 0, c->st, (const double*)a.red, a.rcap, (int)nblk(n),
-                       red_spec({RED_Q(RED_MAX, RED_F64, fhmax), RED_Q(RED_MIN, RED_F64, dtdrag)}), a.sc);
+                       red_spec({RED_Q(RED_MAX, RED_F64, fhmax), RED_Q(RED_MIN, RED_F64, dtdrag)}), a.sc,
+                       (const int*)nullptr);
     hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, c->st, a.sc, c->p);
   }
   HIPCHK(hipGetLastError());
@@ -1987,6 +2237,11 @@ static void fill_outputs(ndsp_ctx* c, const DScal& h, ndsp_stats* stats, ndsp_dt
   }
 }
 
+// Permutes the persistent per-particle state into the cell order of the last
+// binning (slot k <- sorted position k), so that per-particle accesses of the
+// pair kernels stay spatially coherent.  Each field is gathered into scratch
+// and copied back: the field pointers never change, so launch parameters
+// recorded earlier (a captured step graph) stay valid.
 static int reorder(ndsp_ctx* c) {
   KArgs& a = c->a;
   const int n = c->n;
@@ -1994,13 +2249,13 @@ static int reorder(ndsp_ctx* c) {
   for (int fi = 0; fi < F_NPERSIST; fi++) {
     hipLaunchKernelGGL(k_gather_d, dim3(nblk(n)), dim3(BS), 0, c->st, (const double*)a.f[fi], c->scratch_d,
                        (const int*)a.sidx, n);
-    std::swap(a.f[fi], c->scratch_d);
+    hipLaunchKernelGGL(k_copy_d, dim3(nblk(n)), dim3(BS), 0, c->st, (const double*)c->scratch_d, a.f[fi], n);
   }
-  int** ia[3] = {&a.uid, &a.numneigh, &a.nits};
+  int* ia[3] = {a.uid, a.numneigh, a.nits};
   for (int q = 0; q < 3; q++) {
-    hipLaunchKernelGGL(k_gather_i, dim3(nblk(n)), dim3(BS), 0, c->st, (const int*)*ia[q], c->scratch_i,
+    hipLaunchKernelGGL(k_gather_i, dim3(nblk(n)), dim3(BS), 0, c->st, (const int*)ia[q], c->scratch_i,
                        (const int*)a.sidx, n);
-    std::swap(*ia[q], c->scratch_i);
+    hipLaunchKernelGGL(k_copy_i, dim3(nblk(n)), dim3(BS), 0, c->st, (const int*)c->scratch_i, ia[q], n);
   }
   HIPCHK(hipGetLastError());
   c->grid_valid = false;
@@ -2095,6 +2350,7 @@ int ndsp_create(const int32_t* iopt, const double* dopt, int64_t npart, int32_t 
     c->g.nc[d] = 1;
   }
   c->g.ndim = p.ndim;
+  c->reflect = nrefl > 0;
   c->dt = dopt[23];
   c->time = 0.0;
   auto cleanup_fail = [&](int e) { ndsp_destroy(c); return e; };
@@ -2150,7 +2406,9 @@ int ndsp_create(const int32_t* iopt, const double* dopt, int64_t npart, int32_t 
   c->nb_part = 1024;
   ok &= hipMalloc(&c->part, sizeof(double) * 32 * c->nb_part) == hipSuccess;
   ok &= hipMalloc(&c->cons_out, sizeof(double) * 32) == hipSuccess;
+  ok &= hipMalloc((void**)&c->gdev, sizeof(Grid)) == hipSuccess;
   if (!ok) return cleanup_fail(set_err(NDSP_ERR_ALLOC, "hipMalloc buffers"));
+  hipMemcpy(c->gdev, &c->g, sizeof(Grid), hipMemcpyHostToDevice);
   a.tabf = tabf;
   a.tabd = tabd;
   a.n = c->n;
@@ -2159,6 +2417,7 @@ int ndsp_create(const int32_t* iopt, const double* dopt, int64_t npart, int32_t 
   hipMemcpy(tabd, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice);
   DScal hs{};
   hs.dt = c->dt;
+  hs.dt_cur = c->dt;
   hs.dtdrag = DBL_MAX;
   hs.dtcourant = 1.0e6;
   hipMemcpy(a.sc, &hs, sizeof(DScal), hipMemcpyHostToDevice);
@@ -2177,8 +2436,10 @@ int ndsp_destroy(ndsp_ctx* c) {
   void* ptrs[] = {c->scratch_d, c->scratch_i, a.numneigh, a.nits, a.uid, a.cellid, a.sidx, a.P4, a.dxb,
                   a.D4, a.De, a.G0, a.G1, a.G2, a.G3, a.G4, a.G5, a.sc, (void*)a.tabf, (void*)a.tabd,
                   c->part, c->cons_out, a.cstart, a.cfill, c->bsum, c->list_a, c->list_b,
-                  a.nbl, a.nbc, a.nbx, a.hcap, a.red};
+                  a.nbl, a.nbc, a.nbx, a.hcap, a.red, c->gdev};
   for (void* q : ptrs) if (q) hipFree(q);
+  if (c->graph_a) hipGraphExecDestroy(c->graph_a);
+  if (c->graph_b) hipGraphExecDestroy(c->graph_b);
   if (c->mlp_alloced) mlp_free(c->mlp);
   for (auto e : c->ev0) hipEventDestroy(e);
   for (auto e : c->ev1) hipEventDestroy(e);
@@ -2291,6 +2552,7 @@ int ndsp_iterate_density(ndsp_ctx* c, ndsp_stats* stats) {
   if (!c) return set_err(NDSP_ERR_BAD_ARG, "null context");
   if (!c->grid_valid) return set_err(NDSP_ERR_BAD_ARG, "iterate_density: call set_linklist first");
   if (int e = do_density(c)) return e;
+  if (int e = density_finish(c)) return e;
   DScal hs;
   if (int e = sync_scalars(c, hs)) return e;
   timing_collect(c);
@@ -2313,6 +2575,7 @@ int ndsp_derivs(ndsp_ctx* c, ndsp_stats* stats, ndsp_dt* dt) {
   if (!c) return set_err(NDSP_ERR_BAD_ARG, "null context");
   if (int e = do_linklist(c, 0)) return e;
   if (int e = do_density(c)) return e;
+  if (int e = density_finish(c)) return e;
   if (int e = do_rates(c)) return e;
   DScal hs;
   if (int e = sync_scalars(c, hs)) return e;
@@ -2325,39 +2588,43 @@ int ndsp_set_dt(ndsp_ctx* c, double dt, double time) {
   if (!c) return set_err(NDSP_ERR_BAD_ARG, "null context");
   c->dt = dt;
   c->time = time;
+  HIPCHK(hipStreamSynchronize(c->st));
   HIPCHK(hipMemcpy(&c->a.sc->dt, &dt, sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(&c->a.sc->dt_cur, &dt, sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(&c->a.sc->time, &time, sizeof(double), hipMemcpyHostToDevice));
   return 0;
 }
 
-int ndsp_step(ndsp_ctx* c, ndsp_stats* stats, ndsp_dt* dtout) {
-  if (!c) return set_err(NDSP_ERR_BAD_ARG, "null context");
-  if (!c->have_rates) {
-    // the reference evaluates derivs once before the first step
-    // (primitive2conservative, src/conservative2primitive.f90:799)
-    if (int e = ndsp_derivs(c, nullptr, nullptr)) return e;
-  }
-  if (c->grid_valid && c->reorder_every > 0 && c->nsteps % c->reorder_every == 0) {
-    if (int e = reorder(c)) return e;
-  }
-  const double dt = c->dt;
+// First half of a leapfrog step (src/stepND_leapfrog_mhd.f90:107-167):
+// predictor, binning and the first chunk of h-rho passes.
+static int step_part_a(ndsp_ctx* c) {
   {
     TScope t(c, "predict");
-    hipLaunchKernelGGL(k_predict, dim3(nblk(c->n)), dim3(BS), 0, c->st, c->a, c->p, dt);
+    hipLaunchKernelGGL(k_predict, dim3(nblk(c->n)), dim3(BS), 0, c->st, c->a, c->p);
   }
   MlpState& ml = c->mlp;
   if (ml.enabled) {
     TScope t(c, "mlp_seed");
-    hipLaunchKernelGGL(k_mlp_features, dim3(nblk(c->n)), dim3(BS), 0, c->st, c->a, c->p, dt, nexp_of(c->p), ml.X);
+    hipLaunchKernelGGL(k_mlp_features, dim3(nblk(c->n)), dim3(BS), 0, c->st, c->a, c->p, c->dt, nexp_of(c->p),
+                       ml.X);
     if (ml.have_params && ml.steps_seen >= ml.seed_after) mlp_launch_seed(ml, c->n, c->a.f[F_H], c->st);
   }
   if (int e = do_linklist(c, 1)) return e;
   if (int e = do_density(c)) return e;
+  return 0;
+}
+
+// Second half (src/stepND_leapfrog_mhd.f90:167-252): rates, corrector,
+// wrap, next dt; online training of the h predictor when enabled.
+static int step_part_b(ndsp_ctx* c) {
+  MlpState& ml = c->mlp;
   if (ml.enabled) hipLaunchKernelGGL(k_mlp_target, dim3(nblk(c->n)), dim3(BS), 0, c->st, c->a, ml.X);
   if (int e = do_rates(c)) return e;
   {
     TScope t(c, "correct");
-    hipLaunchKernelGGL(k_correct, dim3(nblk(c->n)), dim3(BS), 0, c->st, c->a, c->p, dt);
+    hipLaunchKernelGGL(k_correct, dim3(nblk(c->n)), dim3(BS), 0, c->st, c->a, c->p);
     hipLaunchKernelGGL(k_boundary, dim3(nblk(c->n)), dim3(BS), 0, c->st, c->a, c->p, c->g, 1);
+    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, c->st, c->a.sc);
   }
   if (ml.enabled && ml.train) {
     TScope t(c, "mlp_train");
@@ -2373,15 +2640,78 @@ int ndsp_step(ndsp_ctx* c, ndsp_stats* stats, ndsp_dt* dtout) {
     }
   }
   HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// records one half of a step into an executable graph
+static int capture(ndsp_ctx* c, hipGraphExec_t& exec, int (*part)(ndsp_ctx*)) {
+  if (exec) {
+    hipGraphExecDestroy(exec);
+    exec = nullptr;
+  }
+  hipGraph_t graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+  c->capturing = true;
+  const int e = part(c);
+  c->capturing = false;
+  const hipError_t ce = hipStreamEndCapture(c->st, &graph);
+  if (e) {
+    if (graph) hipGraphDestroy(graph);
+    return e;
+  }
+  if (ce != hipSuccess) return set_err(NDSP_ERR_HIP, std::string("stream capture: ") + hipGetErrorString(ce));
+  const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (ie != hipSuccess) return set_err(NDSP_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+  return 0;
+}
+
+static bool graph_usable(const ndsp_ctx* c) { return c->use_graph && !c->mlp.enabled && !c->timing; }
+
+int ndsp_step(ndsp_ctx* c, ndsp_stats* stats, ndsp_dt* dtout) {
+  if (!c) return set_err(NDSP_ERR_BAD_ARG, "null context");
+  if (!c->have_rates) {
+    // the reference evaluates derivs once before the first step
+    // (primitive2conservative, src/conservative2primitive.f90:799)
+    if (int e = ndsp_derivs(c, nullptr, nullptr)) return e;
+  }
+  if (c->grid_valid && c->reorder_every > 0 && c->nsteps % c->reorder_every == 0) {
+    if (int e = reorder(c)) return e;
+  }
+  if (graph_usable(c) && c->cap_bits > 0) {
+    // the step as two recorded graphs; the h-rho iteration is completed on
+    // the host path in between when it needs more passes than recorded
+    if (!c->graph_valid) {
+      if (int e = capture(c, c->graph_a, step_part_a)) return e;
+      if (int e = capture(c, c->graph_b, step_part_b)) return e;
+      c->graph_valid = true;
+    }
+    HIPCHK(hipGraphLaunch(c->graph_a, c->st));
+    if (int e = density_finish(c)) return e;
+    HIPCHK(hipGraphLaunch(c->graph_b, c->st));
+  } else {
+    if (int e = step_part_a(c)) return e;
+    if (int e = density_finish(c)) return e;
+    if (int e = step_part_b(c)) return e;
+  }
   DScal hs;
   if (int e = sync_scalars(c, hs)) return e;
   timing_collect(c);
-  c->time += dt;
-  c->dt = hs.dt;
+  c->time = hs.time;
+  c->dt = hs.dt_cur;
   c->nsteps++;
-  if (ml.enabled) ml.steps_seen++;
+  if (hs.grid_clamped && c->cap_bits < 28)
+    if (int e = size_key_space(c, hs, 2)) return e;
+  if (c->mlp.enabled) c->mlp.steps_seen++;
   fill_outputs(c, hs, stats, dtout);
   return check_err(c, hs);
+}
+
+int ndsp_set_graph(ndsp_ctx* c, int32_t on) {
+  if (!c) return set_err(NDSP_ERR_BAD_ARG, "null context");
+  c->use_graph = on != 0;
+  c->graph_valid = false;
+  return 0;
 }
 
 int ndsp_conserved(ndsp_ctx* c, double* out) {
@@ -2405,7 +2735,7 @@ int ndsp_neighbours(ndsp_ctx* c, int32_t sym, int32_t* pairs, int64_t cap, int64
   HIPCHK(hipMalloc(&counts, sizeof(int) * n));
   dispatch_ndim(c->p.ndim, [&](auto nd) {
     constexpr int ND = decltype(nd)::value;
-    hipLaunchKernelGGL((k_nb<ND>), dim3(nblk(n)), dim3(BS), 0, c->st, c->a, c->p, c->g, (int)sym,
+    hipLaunchKernelGGL((k_nb<ND>), dim3(nblk(n)), dim3(BS), 0, c->st, c->a, c->p, (const Grid*)c->gdev, (int)sym,
                        (const int*)nullptr, counts);
     return 0;
   });
@@ -2423,7 +2753,7 @@ int ndsp_neighbours(ndsp_ctx* c, int32_t sym, int32_t* pairs, int64_t cap, int64
   HIPCHK(hipMemcpy(offs, ho.data(), sizeof(int) * n, hipMemcpyHostToDevice));
   dispatch_ndim(c->p.ndim, [&](auto nd) {
     constexpr int ND = decltype(nd)::value;
-    hipLaunchKernelGGL((k_nb<ND>), dim3(nblk(n)), dim3(BS), 0, c->st, c->a, c->p, c->g, (int)sym,
+    hipLaunchKernelGGL((k_nb<ND>), dim3(nblk(n)), dim3(BS), 0, c->st, c->a, c->p, (const Grid*)c->gdev, (int)sym,
                        (const int*)offs, dpairs);
     return 0;
   });

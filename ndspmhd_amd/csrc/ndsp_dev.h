@@ -15,6 +15,7 @@
 struct Grid {
   int ndim;
   int nc[3];
+  int mbits;         // bits per dimension of the Morton cell key
   int bound[3];      // 0 open, 2 reflecting, 3 periodic
   double lo[3];      // grid origin
   double w[3];       // cell width
@@ -22,6 +23,31 @@ struct Grid {
   double xmin[3], xmax[3];
   long long ncell;
 };
+
+// Morton (bit-interleaved) key of cell (cx, cy, cz): consecutive keys are
+// mostly adjacent cells, so 64 consecutive particles of the sorted order form
+// a compact group (x bit lowest).
+__host__ __device__ __forceinline__ unsigned spread_bits3(unsigned x) {
+  x &= 0x3ffu;
+  x = (x | (x << 16)) & 0x030000FFu;
+  x = (x | (x << 8)) & 0x0300F00Fu;
+  x = (x | (x << 4)) & 0x030C30C3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+__host__ __device__ __forceinline__ unsigned spread_bits2(unsigned x) {
+  x &= 0xffffu;
+  x = (x | (x << 8)) & 0x00FF00FFu;
+  x = (x | (x << 4)) & 0x0F0F0F0Fu;
+  x = (x | (x << 2)) & 0x33333333u;
+  x = (x | (x << 1)) & 0x55555555u;
+  return x;
+}
+__host__ __device__ __forceinline__ int cell_key(const Grid& g, int cx, int cy, int cz) {
+  if (g.ndim == 3) return (int)(spread_bits3(cx) | (spread_bits3(cy) << 1) | (spread_bits3(cz) << 2));
+  if (g.ndim == 2) return (int)(spread_bits2(cx) | (spread_bits2(cy) << 1));
+  return cx;
+}
 
 // Image of a cell at unwrapped coordinate c along one dimension.
 //   kind 0: inside, 1: periodic image left of xmin (source near xmax),
@@ -122,27 +148,27 @@ __device__ __forceinline__ double mul_exact(double a, double b) {
   return a * b;
 }
 
-// compensated accumulator: FP32 uses the branch-free TwoSum error term
-// (Knuth), FP64 is a plain sum
+// Accumulator of the pair sums.  FP32 pair terms are summed in FP64: on this
+// hardware (no packed FP32 here) an FP64 add costs what an FP32 add costs, so
+// a convert and an add (2 operations) replace a TwoSum compensation (6) and
+// the sum of the FP32 terms is exact to FP64 rounding.  FP64 terms: plain sum.
 template <typename T>
 struct Acc {
-  T s, c;
-  __device__ __forceinline__ void zero() { s = T(0); c = T(0); }
-  __device__ __forceinline__ void add(T x) {
-    T t = s + x;
-    T bp = t - s;
-    c += (s - (t - bp)) + (x - bp);
-    s = t;
-  }
-  __device__ __forceinline__ double value() const { return (double)s + (double)c; }
-};
-template <>
-struct Acc<double> {
   double s;
   __device__ __forceinline__ void zero() { s = 0.0; }
-  __device__ __forceinline__ void add(double x) { s += x; }
+  __device__ __forceinline__ void add(T x) { s += (double)x; }
   __device__ __forceinline__ double value() const { return s; }
 };
+
+// square root, reciprocal and quotient of the pair math: FP32 uses the
+// hardware v_sqrt_f32 / v_rcp_f32 (about 1 ulp), FP64 the correctly rounded
+// operations
+__device__ __forceinline__ float p_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double p_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float p_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double p_rcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float p_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double p_div(double a, double b) { return a / b; }
 
 // linear interpolation in q^2 from a table held in LDS
 // (interpolate_kernels_dens, src/kernelND.f90:4599-4638)
@@ -163,6 +189,20 @@ __device__ __forceinline__ T pown_t(T x) {
   if (N == 1) return x;
   if (N == 2) return x * x;
   return x * x * x;
+}
+
+// Workgroups are dispatched to the 8 XCDs of the MI355X round robin
+// (workgroup b to XCD b mod 8), and each XCD has its own L2.  The pair
+// kernels map workgroup b to tile xcd_tile(b) so that every XCD works on one
+// contiguous range of the (spatially sorted) particles: neighbour gathers of
+// adjacent tiles then hit the same L2.  A bijection of [0, gridDim.x).
+#define NDSP_NXCD 8
+__device__ __forceinline__ int xcd_tile() {
+  const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+  const int x = b % NDSP_NXCD, q = b / NDSP_NXCD;
+  int start = 0;
+  for (int y = 0; y < x; y++) start += (nb - y + NDSP_NXCD - 1) / NDSP_NXCD;
+  return start + q;
 }
 
 // atomics on non-negative doubles through their IEEE bit patterns

@@ -61,6 +61,17 @@ struct DScal {
   int relink;
   int list_invalid;    // some h grew past its list's margin: force pass scans cells
   int list_overflow;   // particles whose list did not fit (they scan cells)
+  int list_on;         // lists built by the last binning (minimum image unique)
+  int grid_clamped;    // cells were widened to fit the key-space capacity
+  double dt_cur;       // dt of the step in progress / the next step
+  // h-rho pass control, advanced on the device by k_pass_ctl
+  int pc_nlist;        // particles in the next pass
+  int pc_its;          // its of the next pass (1 = first)
+  int pc_sym;          // next pass is the symmetric routine
+  int pc_persistent;   // next pass iterates its particles to convergence
+  int pc_done;         // the iteration has finished (or failed)
+  int pc_relink;       // ghost extents were rebuilt before the next pass
+  int pc_refresh;      // 1/h of the packed view must be refreshed
 };
 
 template <typename PT>

@@ -87,6 +87,12 @@ module ndsp_capi
        real(c_double), value :: dt, time
     end function ndsp_set_dt
 
+    integer(c_int) function ndsp_set_graph(ctx, on) bind(c, name='ndsp_set_graph')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: on
+    end function ndsp_set_graph
+
     integer(c_int) function ndsp_conserved(ctx, out) bind(c, name='ndsp_conserved')
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx

@@ -67,6 +67,8 @@ contains
     dt = dopt(24)
     time = 0.
     call ndsp_host_check(ndsp_set_dt(ctx, dt, time), 'ndsp_set_dt')
+    ! steps replay recorded HIP graphs (same results, fewer launches)
+    call ndsp_host_check(ndsp_set_graph(ctx, 1_c_int32_t), 'ndsp_set_graph')
  end subroutine ndsp_host_init
 
  subroutine get_field(name, a, n)
