@@ -75,6 +75,44 @@ def run_gpu(nside, steps, warmup, prec, timing=True, graph=False):
                 ncalc_mean=float(np.mean(nits)), kernel_ms=kt, n=nside ** 3)
 
 
+def run_sweep(nsim, steps):
+    """Batched sweep: nsim independent 1D/2D MHD shock tubes of uneven N
+    (log-uniform 200..20000), all stepped together with ndsp_batch_step, and
+    the same members stepped one after another as the baseline."""
+    import ndspmhd_amd as nd
+    from ndspmhd_amd.setups import shock_sweep
+    cfgs = shock_sweep(nsim=nsim)
+    out = {"members": nsim, "particles": int(sum(st["x"].shape[0] for _, st in cfgs)),
+           "steps": steps}
+    for mode in ("batched", "one_by_one"):
+        sims = [nd.Simulation(o, st, precision=0) for o, st in cfgs]
+        batch = nd.Batch(sims) if mode == "batched" else None
+        for _ in range(2):  # warmup (records the graphs)
+            if batch:
+                batch.step()
+            else:
+                for s in sims:
+                    s.step()
+        pairs = 0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            if batch:
+                batch.step()
+            else:
+                for s in sims:
+                    s.step()
+            pairs += sum(s.stats["pairs_density"] + s.stats["pairs_force"] for s in sims)
+        wall = time.perf_counter() - t0
+        out[mode] = {"sweep_steps_per_s": steps / wall, "pairs_per_s": pairs / wall}
+        if batch:
+            batch.close()
+        for s in sims:
+            s.close()
+    out["speedup_batched"] = out["batched"]["sweep_steps_per_s"] / out["one_by_one"]["sweep_steps_per_s"]
+    log("[bench] sweep of %d: %s" % (nsim, out))
+    return out
+
+
 def field_errors(nside):
     """mixed vs FP64 GPU on one derivs of the same state (norm-relative L-inf)"""
     import ndspmhd_amd as nd
@@ -137,6 +175,9 @@ def main():
     ap.add_argument("--big-nside", type=int, default=256,
                     help="second workload (mixed vs plain FP64 steps/s); 0 skips it")
     ap.add_argument("--big-steps", type=int, default=5)
+    ap.add_argument("--sweep", type=int, default=64,
+                    help="members of the batched shock-tube sweep leg; 0 skips it")
+    ap.add_argument("--sweep-steps", type=int, default=20)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the timed region, one device copy of this many MiB "
                          "(reference kernel for scaling rocprof FETCH_SIZE/WRITE_SIZE)")
@@ -224,6 +265,8 @@ def main():
         big["speedup_mixed_over_fp64"] = big["mixed"]["steps_per_s"] / big["fp64"]["steps_per_s"]
         big["particles"] = nb ** 3
         result["box_%d" % nb] = big
+    if args.sweep > 0:
+        result["sweep"] = run_sweep(args.sweep, args.sweep_steps)
     if args.copy_probe_mib > 0:
         import torch
         nbytes = args.copy_probe_mib << 20

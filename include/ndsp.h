@@ -158,6 +158,18 @@ int ndsp_kernel_times(ndsp_ctx *ctx, double *ms, int32_t cap, int32_t *n,
  * launch-by-launch path.  Not used while timing or the h predictor is on. */
 int ndsp_set_graph(ndsp_ctx *ctx, int32_t on);
 
+/* ---- batches of independent simulations (a parameter sweep; the reference
+ * runs them one after another from multi/ sweep descriptions).  A batch
+ * references existing contexts (it does not own them); ndsp_batch_step takes
+ * one leapfrog step of every member, exactly as ndsp_step would, replaying
+ * both halves of all members as two recorded HIP graphs in which the members
+ * run concurrently.  stats and dt receive one entry per member (may be
+ * NULL).  Members may not time kernels or use the h predictor. */
+typedef struct ndsp_batch ndsp_batch;
+int ndsp_batch_create(ndsp_ctx *const *ctxs, int32_t nsim, ndsp_batch **out);
+int ndsp_batch_destroy(ndsp_batch *batch);
+int ndsp_batch_step(ndsp_batch *batch, ndsp_stats *stats, ndsp_dt *dt);
+
 /* ---- h-predictor MLP (no counterpart in the reference; its seed for the
  * h-rho iteration is h = h_in + dt dh/dt, src/stepND_leapfrog_mhd.f90:153).
  * A 6 -> 32 -> 32 (GELU) -> 1 network, BF16 weights, FP32 accumulation,

@@ -36,7 +36,8 @@ SYMBOLS = ("ndsp_create", "ndsp_destroy", "ndsp_upload", "ndsp_download",
            "ndsp_derivs", "ndsp_step", "ndsp_set_dt", "ndsp_conserved",
            "ndsp_neighbours", "ndsp_set_timing", "ndsp_kernel_times", "ndsp_set_graph",
            "ndsp_mlp_config", "ndsp_mlp_set_params", "ndsp_mlp_get_params",
-           "ndsp_mlp_forward", "ndsp_mlp_grad", "ndsp_mlp_train", "ndsp_last_error")
+           "ndsp_mlp_forward", "ndsp_mlp_grad", "ndsp_mlp_train",
+           "ndsp_batch_create", "ndsp_batch_destroy", "ndsp_batch_step", "ndsp_last_error")
 MLP_NPARAM = 1313
 
 
@@ -82,6 +83,9 @@ def load_library(path: str | None = None):
     L.ndsp_neighbours.argtypes = [vp, ctypes.c_int32, ip, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
     L.ndsp_set_timing.argtypes = [vp, ctypes.c_int32]
     L.ndsp_set_graph.argtypes = [vp, ctypes.c_int32]
+    L.ndsp_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.POINTER(vp)]
+    L.ndsp_batch_destroy.argtypes = [vp]
+    L.ndsp_batch_step.argtypes = [vp, ctypes.POINTER(_Stats), ctypes.POINTER(_Dt)]
     L.ndsp_kernel_times.argtypes = [vp, dp, ctypes.c_int32, ip, ctypes.c_char_p, ctypes.c_int32]
     fp = ctypes.POINTER(ctypes.c_float)
     L.ndsp_mlp_config.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
@@ -251,3 +255,37 @@ class Simulation:
         self._check(L.ndsp_kernel_times(self.h, _dp(ms), 64, ctypes.byref(n), names, 4096))
         labels = names.value.decode().split(",") if n.value else []
         return dict(zip(labels, ms[: n.value]))
+
+
+class Batch:
+    """Independent simulations stepped together (ndsp_batch_*): both halves of
+    every member's leapfrog step replayed as two recorded HIP graphs in which
+    the members run concurrently.  The batch keeps references to the
+    Simulation objects; each member's state, stats and dt_info are updated
+    as by Simulation.step."""
+
+    def __init__(self, sims):
+        L = load_library()
+        self.sims = list(sims)
+        arr = (ctypes.c_void_p * len(self.sims))(*[s.h.value for s in self.sims])
+        h = ctypes.c_void_p()
+        self._check(L.ndsp_batch_create(arr, len(self.sims), ctypes.byref(h)))
+        self.h = h
+        self._st = (_Stats * len(self.sims))()
+        self._dt = (_Dt * len(self.sims))()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise NdspError(rc, load_library().ndsp_last_error().decode())
+
+    def step(self):
+        self._check(load_library().ndsp_batch_step(self.h, self._st, self._dt))
+        for k, sim in enumerate(self.sims):
+            sim._stats(self._st[k], self._dt[k])
+
+    def close(self):
+        if getattr(self, "h", None):
+            load_library().ndsp_batch_destroy(self.h)
+            self.h = None
+
+    __del__ = close

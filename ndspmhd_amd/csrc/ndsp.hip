@@ -370,7 +370,7 @@ __global__ void k_pack_force(KArgs a, Phys p) {
 // Calls body(kk, xj_img[3], flip[3], code) for every particle of every cell
 // image that can hold a point within rad of xi; xj_img holds the image
 // coordinates (ghost arithmetic of src/ghostND_mhd.f90:180-200), flip[d] = 1
-// where a reflecting image reverses the velocity component, code = the image
+// where the image's velocity component is reversed (image_flip), code = the image
 // kinds kx + 5 ky + 25 kz (0 for a particle seen in place).
 template <int NDIM, typename F>
 __device__ __forceinline__ void for_each_candidate(const KArgs& a, const Grid& g, const double* xi,
@@ -410,7 +410,8 @@ __device__ __forceinline__ void for_each_candidate(const KArgs& a, const Grid& g
         if (sx < 0) continue;
         const int kind[3] = {kx, ky, kz};
         const int code = kx + 5 * ky + 25 * kz;
-        const int fl[3] = {kx >= 3, ky >= 3, kz >= 3};
+        int fl[3];
+        image_flip(kind, fl);
         const int key = cell_key(g, sx, sy, sz);
         const int b = a.cstart[key], e = a.cstart[key + 1];
         for (int kk = b; kk < e; kk++) {
@@ -558,9 +559,9 @@ __device__ __forceinline__ int wave_max_i(int v) {
 #define NBL_CELLS 256
 template <int NDIM>
 __global__ __launch_bounds__(64) void k_nblist(KArgs a, Phys p, const Grid* __restrict__ gp) {
-  __shared__ double4 sraw[64];
-  __shared__ float4 srel[64];
-  __shared__ unsigned se[64];
+  __shared__ double4 sraw[128];
+  __shared__ float4 srel[128];
+  __shared__ unsigned se[128];
   __shared__ int ckey[NBL_CELLS];
   __shared__ int ccode[NBL_CELLS];
   __shared__ int cbeg[NBL_CELLS];
@@ -589,17 +590,43 @@ __global__ __launch_bounds__(64) void k_nblist(KArgs a, Phys p, const Grid* __re
   unsigned* L = a.nbl + (size_t)(act ? k : 0) * a.maxnb;
   int cnt = 0;
   bool over = false;
+  // box of the group's particles widened by the largest search radius
+  double gbox_lo[3], gbox_hi[3];
+  {
+    double rw = rad;
+    for (int o = 32; o > 0; o >>= 1) rw = fmax(rw, __shfl_xor(rw, o));
+    rw *= 1.0 + 1.0e-9;
+    for (int d = 0; d < 3; d++) {
+      double lo_ = act ? xi[d] : DBL_MAX, hi_ = act ? xi[d] : -DBL_MAX;
+      for (int o = 32; o > 0; o >>= 1) {
+        lo_ = fmin(lo_, __shfl_xor(lo_, o));
+        hi_ = fmax(hi_, __shfl_xor(hi_, o));
+      }
+      gbox_lo[d] = lo_ - rw;
+      gbox_hi[d] = hi_ + rw;
+    }
+  }
+  int fill = 0;  // staged candidates awaiting the test (wave-uniform)
 
   // test the staged candidates [0, m) against this lane's particle
+  const float rl2f = lim2f / hi2f;  // squared list radius of this particle
   auto test = [&](int m) {
     if (!act) return;
     unsigned long long mask = 0ull;
-    for (int t = 0; t < m; t++) {
-      const float4 c = srel[t];
-      const float dx = xr0 - c.x, dy = xr1 - c.y, dz = xr2 - c.z;
-      const float r2f = dx * dx + dy * dy + dz * dz;
-      if (r2f * hi2f < lim2f) mask |= 1ull << t;
+#pragma unroll 1
+    for (int t0 = 0; t0 < m; t0 += 8) {
+      // 8 candidates with a constant bit each (entries past m are masked)
+      unsigned m8 = 0u;
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const float4 c = srel[t0 + u];
+        const float dx = xr0 - c.x, dy = xr1 - c.y, dz = xr2 - c.z;
+        const float r2f = dx * dx + dy * dy + dz * dz;
+        m8 |= r2f < rl2f ? (1u << u) : 0u;
+      }
+      mask |= (unsigned long long)m8 << t0;
     }
+    if (m < 64) mask &= (1ull << m) - 1ull;
     while (mask) {
       const int t = __builtin_ctzll(mask);
       mask &= mask - 1ull;
@@ -662,23 +689,38 @@ __global__ __launch_bounds__(64) void k_nblist(KArgs a, Phys p, const Grid* __re
     unsigned ent = 0u;
     if (lane < T) fetch(lane, pj, ent);
     for (int q0 = 0; q0 < T; q0 += 64) {
-      const int m = min(64, T - q0);
-      if (lane < m) {
-        const int code = (int)(ent & 63u);
-        double xj[3] = {pj.x, pj.y, pj.z};
-        for (int d = 0; d < NDIM; d++) {
-          const int kd = (code >> (2 * d)) & 3;
-          if (kd) image_coord(g, d, kd, xj[d], 0.0, xj[d]);
-        }
-        sraw[lane] = pj;
-        srel[lane] = make_float4((float)(xj[0] - org[0]), NDIM >= 2 ? (float)(xj[1] - org[1]) : 0.f,
-                                 NDIM >= 3 ? (float)(xj[2] - org[2]) : 0.f, 0.f);
-        se[lane] = ent;
+      // keep the candidates inside the group's box widened by the largest
+      // radius, compacted in order behind the ones already staged
+      const int code = (int)(ent & 63u);
+      double xj[3] = {pj.x, pj.y, pj.z};
+      for (int d = 0; d < NDIM; d++) {
+        const int kd = (code >> (2 * d)) & 3;
+        if (kd) image_coord(g, d, kd, xj[d], 0.0, xj[d]);
       }
-      __syncthreads();
+      bool keep = lane < T - q0;
+      for (int d = 0; d < NDIM; d++) keep = keep && xj[d] >= gbox_lo[d] && xj[d] <= gbox_hi[d];
+      const unsigned long long bal = __ballot(keep);
+      if (keep) {
+        const int pos = fill + __popcll(bal & ((1ull << lane) - 1ull));
+        sraw[pos] = pj;
+        srel[pos] = make_float4((float)(xj[0] - org[0]), NDIM >= 2 ? (float)(xj[1] - org[1]) : 0.f,
+                                NDIM >= 3 ? (float)(xj[2] - org[2]) : 0.f, 0.f);
+        se[pos] = ent;
+      }
+      fill += __popcll(bal);
       if (q0 + 64 + lane < T) fetch(q0 + 64 + lane, pj, ent);  // next 64, in flight during the test
-      test(m);
-      __syncthreads();
+      if (fill >= 64) {
+        __syncthreads();
+        test(64);
+        __syncthreads();
+        if (lane < fill - 64) {
+          sraw[lane] = sraw[64 + lane];
+          srel[lane] = srel[64 + lane];
+          se[lane] = se[64 + lane];
+        }
+        fill -= 64;
+        __syncthreads();
+      }
     }
     ncl = 0;
   };
@@ -726,6 +768,10 @@ __global__ __launch_bounds__(64) void k_nblist(KArgs a, Phys p, const Grid* __re
     }
   }
   if (ncl) process_batch();
+  if (fill) {
+    __syncthreads();
+    test(fill);
+  }
   if (act) {
     a.nbc[k] = over ? -1 : cnt;
     a.hcap[k] = s1 / pk.w;
@@ -1258,7 +1304,9 @@ __global__ __launch_bounds__(BS) void k_force(KArgs a, Phys p, const Grid* __res
       r.g0 = G0[kk];
       r.g1 = G1[kk];
       r.g2 = G2[kk];
-      r.g4 = G4[kk];
+      // (h, EOS density) of a gas without dust are 1/P4.w and rho: no load
+      if (dust) r.g4 = G4[kk];
+      else r.g4 = T4{PT(0), PT(0), PT(0), PT(0)};
       if (mhd) r.g3 = G3[kk];
       else r.g3 = T4{PT(0), PT(0), PT(0), PT(0)};
       r.g5 = dust ? G5[kk] : PT(0);
@@ -1294,7 +1342,7 @@ __global__ __launch_bounds__(BS) void k_force(KArgs a, Phys p, const Grid* __res
       const PT prj = fmax(j1.y, PT(0));
       const PT Prho2j = prj * rho21j;
       const PT csj = j1.z;
-      const PT hj = j4.x;
+      const PT hj = dust ? j4.x : p_rcp(hj1p);
       PT Bj[3] = {0, 0, 0}, psij = 0, dB[3] = {0, 0, 0};
       PT projBi = 0, projBj = 0, projdB = 0, projBrhoi = 0, projBrhoj = 0, Brho2j = 0, valfven2j = 0;
       PT Brhoj[3] = {0, 0, 0};
@@ -1308,7 +1356,7 @@ __global__ __launch_bounds__(BS) void k_force(KArgs a, Phys p, const Grid* __res
         projBrhoi = Brhoi[0] * dr[0] + Brhoi[1] * dr[1] + Brhoi[2] * dr[2];
         projBrhoj = Brhoj[0] * dr[0] + Brhoj[1] * dr[1] + Brhoj[2] * dr[2];
         const PT B2j = Bj[0] * Bj[0] + Bj[1] * Bj[1] + Bj[2] * Bj[2];
-        valfven2j = p_div(B2j, j4.y);
+        valfven2j = p_div(B2j, dust ? j4.y : rhoj);
         Brho2j = B2j * rho21j;
       }
       // signal velocities, src/ratesND_mhd.f90:1417-1482
@@ -2705,6 +2753,137 @@ int ndsp_step(ndsp_ctx* c, ndsp_stats* stats, ndsp_dt* dtout) {
   if (c->mlp.enabled) c->mlp.steps_seen++;
   fill_outputs(c, hs, stats, dtout);
   return check_err(c, hs);
+}
+
+// ===========================================================================
+// batches of independent simulations (a parameter sweep): one recorded graph
+// per half step covering every member, members on their own streams
+// ===========================================================================
+
+struct ndsp_batch {
+  std::vector<ndsp_ctx*> sims;
+  hipStream_t st = nullptr;          // stream the graphs are launched on
+  hipGraphExec_t ga = nullptr, gb = nullptr;
+  bool valid = false;
+  DScal* hs = nullptr;               // pinned host copies of every member's scalars
+  std::vector<hipEvent_t> ev;        // fork/join events of the recording
+  std::vector<int> caps;             // members' key-space sizes at recording
+};
+
+// records part(c) of every member into one graph: each member's launches go
+// to its own stream (forked from and joined back to the batch stream) and
+// end with a copy of its scalars into the pinned host array
+static int batch_capture(ndsp_batch* b, hipGraphExec_t& exec, int (*part)(ndsp_ctx*)) {
+  if (exec) {
+    hipGraphExecDestroy(exec);
+    exec = nullptr;
+  }
+  const int ns = (int)b->sims.size();
+  hipGraph_t graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+  int err = 0;
+  HIPCHK(hipEventRecord(b->ev[0], b->st));
+  for (int s = 0; s < ns && !err; s++) {
+    ndsp_ctx* c = b->sims[s];
+    if (hipStreamWaitEvent(c->st, b->ev[0], 0) != hipSuccess) { err = set_err(NDSP_ERR_HIP, "fork"); break; }
+    c->capturing = true;
+    err = part(c);
+    c->capturing = false;
+    if (err) break;
+    if (hipMemcpyAsync(&b->hs[s], c->a.sc, sizeof(DScal), hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+        hipEventRecord(b->ev[1 + s], c->st) != hipSuccess || hipStreamWaitEvent(b->st, b->ev[1 + s], 0) != hipSuccess)
+      err = set_err(NDSP_ERR_HIP, "join");
+  }
+  const hipError_t ce = hipStreamEndCapture(b->st, &graph);
+  if (err) {
+    if (graph) hipGraphDestroy(graph);
+    return err;
+  }
+  if (ce != hipSuccess) return set_err(NDSP_ERR_HIP, std::string("batch capture: ") + hipGetErrorString(ce));
+  const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (ie != hipSuccess) return set_err(NDSP_ERR_HIP, std::string("batch instantiate: ") + hipGetErrorString(ie));
+  return 0;
+}
+
+int ndsp_batch_create(ndsp_ctx* const* ctxs, int32_t nsim, ndsp_batch** out) {
+  if (!ctxs || !out || nsim < 1) return set_err(NDSP_ERR_BAD_ARG, "batch_create: bad arguments");
+  *out = nullptr;
+  for (int s = 0; s < nsim; s++)
+    if (!ctxs[s]) return set_err(NDSP_ERR_BAD_ARG, "batch_create: null member");
+  ndsp_batch* b = new ndsp_batch();
+  b->sims.assign(ctxs, ctxs + nsim);
+  bool ok = hipStreamCreate(&b->st) == hipSuccess;
+  ok &= hipHostMalloc((void**)&b->hs, sizeof(DScal) * nsim, hipHostMallocDefault) == hipSuccess;
+  b->ev.resize(nsim + 1, nullptr);
+  for (auto& e : b->ev) ok &= hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    ndsp_batch_destroy(b);
+    return set_err(NDSP_ERR_ALLOC, "batch_create: stream, events or pinned memory");
+  }
+  *out = b;
+  return 0;
+}
+
+int ndsp_batch_destroy(ndsp_batch* b) {
+  if (!b) return 0;
+  if (b->st) hipStreamSynchronize(b->st);
+  if (b->ga) hipGraphExecDestroy(b->ga);
+  if (b->gb) hipGraphExecDestroy(b->gb);
+  for (auto e : b->ev)
+    if (e) hipEventDestroy(e);
+  if (b->hs) hipHostFree(b->hs);
+  if (b->st) hipStreamDestroy(b->st);
+  delete b;
+  return 0;
+}
+
+// One leapfrog step of every member (each as ndsp_step would take it): the
+// two halves of all members are replayed as two graphs, members running
+// concurrently on their own streams; members whose h-rho iteration needs
+// more passes than recorded complete it on the host path in between.
+int ndsp_batch_step(ndsp_batch* b, ndsp_stats* stats, ndsp_dt* dt) {
+  if (!b) return set_err(NDSP_ERR_BAD_ARG, "null batch");
+  const int ns = (int)b->sims.size();
+  for (int s = 0; s < ns; s++) {
+    ndsp_ctx* c = b->sims[s];
+    if (c->mlp.enabled || c->timing)
+      return set_err(NDSP_ERR_BAD_OPTION, "batch_step: members may not time kernels or use the h predictor");
+    if (!c->have_rates)
+      if (int e = ndsp_derivs(c, nullptr, nullptr)) return e;
+    if (c->grid_valid && c->reorder_every > 0 && c->nsteps % c->reorder_every == 0)
+      if (int e = reorder(c)) return e;
+    if (b->caps.size() != (size_t)ns || b->caps[s] != c->cap_bits) b->valid = false;
+  }
+  for (int s = 0; s < ns; s++) HIPCHK(hipStreamSynchronize(b->sims[s]->st));
+  if (!b->valid) {
+    if (int e = batch_capture(b, b->ga, step_part_a)) return e;
+    if (int e = batch_capture(b, b->gb, step_part_b)) return e;
+    b->caps.resize(ns);
+    for (int s = 0; s < ns; s++) b->caps[s] = b->sims[s]->cap_bits;
+    b->valid = true;
+  }
+  HIPCHK(hipGraphLaunch(b->ga, b->st));
+  HIPCHK(hipStreamSynchronize(b->st));
+  for (int s = 0; s < ns; s++)
+    if (!b->hs[s].pc_done)
+      if (int e = density_finish(b->sims[s])) return e;
+  HIPCHK(hipGraphLaunch(b->gb, b->st));
+  HIPCHK(hipStreamSynchronize(b->st));
+  int first_err = 0;
+  for (int s = 0; s < ns; s++) {
+    ndsp_ctx* c = b->sims[s];
+    const DScal hs = b->hs[s];
+    c->time = hs.time;
+    c->dt = hs.dt_cur;
+    c->nsteps++;
+    if (hs.grid_clamped && c->cap_bits < 28)
+      if (int e = size_key_space(c, hs, 2)) return e;
+    fill_outputs(c, hs, stats ? stats + s : nullptr, dt ? dt + s : nullptr);
+    const int e = check_err(c, hs);
+    if (e && !first_err) first_err = e;
+  }
+  return first_err;
 }
 
 int ndsp_set_graph(ndsp_ctx* c, int32_t on) {

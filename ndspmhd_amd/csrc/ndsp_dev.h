@@ -105,6 +105,24 @@ __device__ __forceinline__ bool image_coord(const Grid& g, int d, int kind, doub
   }
 }
 
+// Velocity of an image as the reference's ghost loop leaves it
+// (src/ghostND_mhd.f90:153-330, at most one reflecting dimension): a ghost
+// is made while processing its highest imaged dimension c, whose reflection
+// flips v_c; an edge ghost whose lower imaged dimension reflects gets the
+// unflipped velocity (:281-282); a 3D corner ghost whose lowest dimension
+// reflects gets -v in every component (:305-306).  flip[d] = 1 negates v_d.
+__host__ __device__ __forceinline__ void image_flip(const int* kind, int* flip) {
+  flip[0] = flip[1] = flip[2] = 0;
+  int nk = 0, c = -1, r = -1;
+  for (int d = 0; d < 3; d++) {
+    if (kind[d]) { nk++; c = d; }
+    if (kind[d] >= 3) r = d;
+  }
+  if (r < 0) return;
+  if (r == c) flip[r] = 1;
+  else if (nk == 3 && r == 0) flip[0] = flip[1] = flip[2] = 1;
+}
+
 // unwrapped cell range covering [x - rad, x + rad] along d (a small margin on
 // both sides keeps every true neighbour in range; the exact FP64 distance test
 // decides membership)

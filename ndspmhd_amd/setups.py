@@ -160,3 +160,75 @@ def brio_wu_1d(npart_target: int = 800, hfact: float = 1.2,
     st["alpha"][:, 1] = o.alphaumin
     st["alpha"][:, 2] = o.alphaBmin
     return o, st
+
+
+def shock_tube_2d(npart_target: int = 2000, hfact: float = 1.2, states=None,
+                  gamma: float = 2.0, width_rows: int = 24, opts: Options | None = None):
+    """2D MHD shock tube: x in [-0.5, 0.5] with the Brio-Wu states of
+    brio_wu_1d, a periodic y extent of `width_rows` left-state lattice
+    spacings (src/setup_shockND.f90:211-215), equal particle masses (the
+    right lattice is sparser by rhoL/rhoR in area).  The x walls reflect
+    (ibound 2); the reference's 2D tube holds the x-boundary particles fixed
+    (ibound 1), which this library does not implement."""
+    rhoL, rhoR, pL, pR, ByL, ByR, Bx = states or (1.0, 0.125, 1.0, 0.1, 1.0, -1.0, 0.75)
+    # left lattice: nxL x ny over [-0.5, 0] x [0, Ly]
+    area_frac = rhoL / (rhoL + rhoR)
+    nL_target = max(8, int(round(npart_target * area_frac)))
+    ny = width_rows
+    nxL = max(2, int(round(nL_target / ny)))
+    psepL = 0.5 / nxL
+    Ly = ny * psepL
+    m = rhoL * psepL * psepL
+    # right lattice: same mass, spacing scaled by sqrt(rhoL/rhoR), rows fit Ly
+    nyR = max(2, int(round(ny * np.sqrt(rhoR / rhoL))))
+    psepRy = Ly / nyR
+    psepRx = m / (rhoR * psepRy)
+    nxR = max(2, int(round(0.5 / psepRx)))
+    psepRx = 0.5 / nxR
+    gx, gy = np.meshgrid((np.arange(nxL) + 0.5) * psepL - 0.5, (np.arange(ny) + 0.5) * psepL)
+    hx, hy = np.meshgrid((np.arange(nxR) + 0.5) * psepRx, (np.arange(nyR) + 0.5) * psepRy)
+    x = np.concatenate([np.stack([gx.ravel(), gy.ravel()], 1), np.stack([hx.ravel(), hy.ravel()], 1)])
+    nL = nxL * ny
+    n = x.shape[0]
+    st = empty_state(n, 2)
+    st["x"][:] = x
+    mR = rhoR * psepRx * psepRy
+    st["pmass"][:nL] = m
+    st["pmass"][nL:] = mR
+    rho = np.concatenate([np.full(nL, rhoL), np.full(n - nL, rhoR)])
+    pr = np.concatenate([np.full(nL, pL), np.full(n - nL, pR)])
+    st["hh"][:] = _h_from_rho(st["pmass"], rho, hfact, 2)
+    st["en"][:] = pr / ((gamma - 1.0) * rho)
+    st["Bevol"][:, 0] = Bx
+    st["Bevol"][:, 1] = np.concatenate([np.full(nL, ByL), np.full(n - nL, ByR)])
+    o = opts or Options()
+    o = o.replace(ndim=2, ikernel=0, iener=2, imhd=11, imagforce=2,
+                  idivbzero=2, iav=2, iavlim=(2, 1, 0), idust=0,
+                  ibound=(2, 3, 0), hfact=hfact, gamma=gamma,
+                  xmin=(-0.5, 0.0, 0.0), xmax=(0.5, Ly, 0.0), psep=psepL)
+    st["alpha"][:, 0] = o.alphamin
+    st["alpha"][:, 1] = o.alphaumin
+    st["alpha"][:, 2] = o.alphaBmin
+    return o, st
+
+
+def shock_sweep(nsim: int = 64, seed: int = 4, nmin: int = 200, nmax: int = 20000):
+    """The batched sweep: nsim independent MHD shock tubes, each seeded
+    seed + k: N drawn log-uniform in [nmin, nmax], dimension 1 or 2, hfact
+    uniform in [1.0, 1.5], gamma 5/3 or 2, and the Brio-Wu left/right
+    densities, pressures and fields scaled by factors uniform in [0.5, 2]
+    (the sweeps of multi/multi_shock.f90).  Returns [(opts, state), ...]."""
+    sims = []
+    for k in range(nsim):
+        rng = np.random.default_rng(seed + k)
+        n = int(round(np.exp(rng.uniform(np.log(nmin), np.log(nmax)))))
+        ndim = int(rng.integers(1, 3))
+        hfact = float(rng.uniform(1.0, 1.5))
+        gamma = 5.0 / 3.0 if rng.integers(0, 2) == 0 else 2.0
+        f = rng.uniform(0.5, 2.0, size=7)
+        states = (1.0 * f[0], 0.125 * f[1], 1.0 * f[2], 0.1 * f[3], 1.0 * f[4], -1.0 * f[5], 0.75 * f[6])
+        if ndim == 1:
+            sims.append(brio_wu_1d(n, hfact=hfact, states=states, gamma=gamma))
+        else:
+            sims.append(shock_tube_2d(n, hfact=hfact, states=states, gamma=gamma))
+    return sims

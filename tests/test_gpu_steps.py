@@ -63,3 +63,21 @@ def test_brio_wu_1d_steps(prec, tol):
         err = np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
         print(k, err, flush=True)
         assert err <= (tol if k != "x" else tol * 1e-2), (k, err)
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-9), (0, 1e-4)])
+def test_shock_tube_2d_steps(prec, tol):
+    """reflecting x walls with a periodic y: edge images follow the
+    reference's ghost velocities (src/ghostND_mhd.f90:281-282)"""
+    from ndspmhd_amd.setups import shock_tube_2d
+    o, st = shock_tube_2d(600, hfact=1.2)
+    print("2d shock tube steps prec", prec, flush=True)
+    sim, orc = _run(o, st, 6, prec)
+    bad = []
+    for k in ("x", "vel", "Bevol", "en", "hh", "rho", "force", "dBevoldt"):
+        a, b = sim.get(k), orc.get(k)
+        err = np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+        print(k, err, flush=True)
+        if err > (tol if k != "x" else tol * 1e-2):
+            bad.append((k, err))
+    assert not bad, bad
