@@ -61,6 +61,7 @@ def run_gpu(nside, steps, warmup, prec, timing=True, graph=False):
     for _ in range(warmup):
         sim.step()
     sim.set_timing(timing)
+    cons0 = sim.conserved()
     pd = pf = 0
     nits = []
     t0 = time.perf_counter()
@@ -71,8 +72,81 @@ def run_gpu(nside, steps, warmup, prec, timing=True, graph=False):
         nits.append(sim.stats["ncalc_mean"])
     t1 = time.perf_counter()
     kt = sim.kernel_times() if timing else {}
+    cons1 = sim.conserved()
     return dict(sim=sim, opts=o, wall=t1 - t0, pairs_density=pd, pairs_force=pf,
-                ncalc_mean=float(np.mean(nits)), kernel_ms=kt, n=nside ** 3)
+                ncalc_mean=float(np.mean(nits)), kernel_ms=kt, n=nside ** 3,
+                cons=(cons0, cons1, float(np.sum(st["pmass"]))))
+
+
+def conservation(cons0, cons1, mass):
+    """drift of the FP64 conserved sums over the timed steps: relative
+    change of the total energy; change of the total momentum vector and of
+    the angular momentum, in units of sqrt(2 M E_kin) (the momentum scale of
+    the flow)"""
+    pscale = np.sqrt(2.0 * mass * max(cons0["ekin"], 1e-300))
+    dmom = np.sqrt(sum((cons1[k] - cons0[k]) ** 2 for k in ("momx", "momy", "momz")))
+    dang = np.sqrt(sum((cons1[k] - cons0[k]) ** 2 for k in ("angx", "angy", "angz")))
+    return {"etot_rel_drift": (cons1["etot"] - cons0["etot"]) / abs(cons0["etot"]),
+            "momentum_drift": dmom / pscale, "angular_momentum_drift": dang / pscale,
+            "etot_start": cons0["etot"], "etot_end": cons1["etot"]}
+
+
+def traffic_from_profile(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, each scaled by
+    the ratio of known bytes to counted bytes of the plain copy kernel);
+    None when no summary is committed"""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_mhd128_mixed.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] in ("fetch_kib", "write_kib"):
+            vals[(f[0], f[1])] = float(f[2])
+    try:
+        cf = vals[("fetch_kib", "k_copy_d")] * 1024.0 / vals[("fetch_kib", "copy_bytes")]
+        cw = vals[("write_kib", "k_copy_d")] * 1024.0 / vals[("write_kib", "copy_bytes")]
+        return (vals[("fetch_kib", kernel)] * 1024.0 / cf + vals[("write_kib", kernel)] * 1024.0 / cw)
+    except KeyError:
+        return None
+
+
+def run_mlp(nside, steps):
+    """h-predictor leg: the same box stepped with the reference seed
+    h_in + dt dh/dt (mixed precision), with the online-trained MLP seed
+    (mixed), and with the reference seed in plain FP64; reports the mean
+    Newton-Raphson passes per particle over the second half of the run and
+    the deviation of h and rho of the MLP run from the plain FP64 run (both
+    iterate every particle to tolh)."""
+    import ndspmhd_amd as nd
+    o, st = setup(nside)
+    out = {"particles": nside ** 3, "steps": steps, "tolh": o.tolh}
+    fields = {}
+    for mode, prec in (("plain", 0), ("mlp", 0), ("plain_fp64", 1)):
+        sim = nd.Simulation(o, st, precision=prec)
+        if mode == "mlp":
+            sim.mlp_config(enable=True, train=True, lr=2e-3, batch=16384, iters=4, seed_after=3)
+        sim.derivs()
+        nc = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sim.step()
+            nc.append(sim.stats["ncalc_mean"])
+        wall = time.perf_counter() - t0
+        out[mode] = {"ncalc_mean_second_half": float(np.mean(nc[steps // 2:])), "steps_per_s": steps / wall}
+        fields[mode] = (sim.get("hh").copy(), sim.get("rho").copy())
+        sim.close()
+    for k, (a, b) in {"h": (fields["mlp"][0], fields["plain_fp64"][0]),
+                      "rho": (fields["mlp"][1], fields["plain_fp64"][1])}.items():
+        out["max_rel_dev_vs_fp64_" + k] = float(np.max(np.abs(a - b) / np.abs(b)))
+    for k, (a, b) in {"h": (fields["plain"][0], fields["plain_fp64"][0]),
+                      "rho": (fields["plain"][1], fields["plain_fp64"][1])}.items():
+        out["plain_max_rel_dev_vs_fp64_" + k] = float(np.max(np.abs(a - b) / np.abs(b)))
+    e_p, e_m = out["plain"]["ncalc_mean_second_half"] - 1.0, out["mlp"]["ncalc_mean_second_half"] - 1.0
+    out["extra_passes_saved_fraction"] = (e_p - e_m) / e_p if e_p > 0 else 0.0
+    log("[bench] h predictor on %d^3: %s" % (nside, out))
+    return out
 
 
 def run_sweep(nsim, steps):
@@ -84,9 +158,16 @@ def run_sweep(nsim, steps):
     cfgs = shock_sweep(nsim=nsim)
     out = {"members": nsim, "particles": int(sum(st["x"].shape[0] for _, st in cfgs)),
            "steps": steps}
-    for mode in ("batched", "one_by_one"):
+    for mode in ("batched_tuned", "batched", "one_by_one"):
         sims = [nd.Simulation(o, st, precision=0) for o, st in cfgs]
-        batch = nd.Batch(sims) if mode == "batched" else None
+        batch = nd.Batch(sims) if mode != "one_by_one" else None
+        if mode == "batched_tuned":
+            t0 = time.perf_counter()
+            choices = batch.autotune(reps=2)
+            out["autotune_s"] = time.perf_counter() - t0
+            out["tile_shapes"] = {"force_workgroup_64": sum(c["force_workgroup"] == 64 for c in choices),
+                                  "density_workgroup_64": sum(c["density_workgroup"] == 64 for c in choices),
+                                  "cells_per_radius_1": sum(c["cells_per_radius"] == 1 for c in choices)}
         for _ in range(2):  # warmup (records the graphs)
             if batch:
                 batch.step()
@@ -109,6 +190,7 @@ def run_sweep(nsim, steps):
         for s in sims:
             s.close()
     out["speedup_batched"] = out["batched"]["sweep_steps_per_s"] / out["one_by_one"]["sweep_steps_per_s"]
+    out["speedup_batched_tuned"] = out["batched_tuned"]["sweep_steps_per_s"] / out["one_by_one"]["sweep_steps_per_s"]
     log("[bench] sweep of %d: %s" % (nsim, out))
     return out
 
@@ -175,6 +257,9 @@ def main():
     ap.add_argument("--big-nside", type=int, default=256,
                     help="second workload (mixed vs plain FP64 steps/s); 0 skips it")
     ap.add_argument("--big-steps", type=int, default=5)
+    ap.add_argument("--mlp-nside", type=int, default=256,
+                    help="box of the h-predictor leg (plain seed vs trained seed); 0 skips it")
+    ap.add_argument("--mlp-steps", type=int, default=12)
     ap.add_argument("--sweep", type=int, default=64,
                     help="members of the batched shock-tube sweep leg; 0 skips it")
     ap.add_argument("--sweep-steps", type=int, default=20)
@@ -228,12 +313,13 @@ def main():
                                "gamma 5/3, v_rms Mach 1 P(k)~k^-4, B=(0.1,0,0)" % (args.nside, n),
                    "particles": n, "steps": args.steps},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile("k_" + dom),
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": dom_ms,
                      "force_fp32_tflops": flops / (force_ms * 1e-3) / 1e12 if force_ms == force_ms else None},
         "kernel_ms": kt,
         "pairs_per_step": {"density": r["pairs_density"] / args.steps, "force": pairs_force_step},
         "ncalc_mean": r["ncalc_mean"],
+        "conservation": conservation(*r["cons"]),
     }
     r["sim"].close()
     log("[bench] same workload, steps replayed as HIP graphs")
@@ -265,17 +351,17 @@ def main():
         big["speedup_mixed_over_fp64"] = big["mixed"]["steps_per_s"] / big["fp64"]["steps_per_s"]
         big["particles"] = nb ** 3
         result["box_%d" % nb] = big
+    if args.mlp_nside > 0 and not args.no_fp64:
+        result["mlp_seed"] = run_mlp(args.mlp_nside, args.mlp_steps)
     if args.sweep > 0:
         result["sweep"] = run_sweep(args.sweep, args.sweep_steps)
     if args.copy_probe_mib > 0:
-        import torch
-        nbytes = args.copy_probe_mib << 20
-        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        result["copy_probe_bytes"] = nbytes
-        del src, dst
+        import ndspmhd_amd as nd
+        o, st = setup(args.nside)
+        probe = nd.Simulation(o, st, precision=0)
+        probe.copy_probe(args.copy_probe_mib << 20)
+        result["copy_probe_bytes"] = args.copy_probe_mib << 20
+        probe.close()
     if not args.no_cpu:
         log("[bench] CPU baseline on %d^3" % args.cpu_nside)
         result["cpu_baseline"] = cpu_baseline(args.cpu_nside)

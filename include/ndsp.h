@@ -150,6 +150,18 @@ int ndsp_set_timing(ndsp_ctx *ctx, int32_t on);
 int ndsp_kernel_times(ndsp_ctx *ctx, double *ms, int32_t cap, int32_t *n,
                       char *names, int32_t names_cap);
 
+/* Autotuner (no counterpart in the reference): times `reps` derivs
+ * evaluations for each tile shape -- workgroup size of the force and h-rho
+ * kernels (64 or 256) and cells per search radius of the binning (1 or 2) --
+ * and keeps the fastest; choice[3] (may be NULL) receives them.  The evolved
+ * state is restored afterwards, so results do not change. */
+int ndsp_autotune(ndsp_ctx *ctx, int32_t reps, int32_t *choice);
+
+/* Test support (no counterpart in the reference): copies nbytes between two
+ * fresh device buffers with the library's plain copy kernel, a reference of
+ * known traffic for calibrating hardware memory counters. */
+int ndsp_copy_probe(ndsp_ctx *ctx, int64_t nbytes);
+
 /* Step graphs (no counterpart in the reference, whose step is a host loop):
  * with on != 0, ndsp_step records its two halves (predictor + binning +
  * first h-rho passes; rates + corrector + next dt) into HIP graphs once and

@@ -37,7 +37,9 @@ SYMBOLS = ("ndsp_create", "ndsp_destroy", "ndsp_upload", "ndsp_download",
            "ndsp_neighbours", "ndsp_set_timing", "ndsp_kernel_times", "ndsp_set_graph",
            "ndsp_mlp_config", "ndsp_mlp_set_params", "ndsp_mlp_get_params",
            "ndsp_mlp_forward", "ndsp_mlp_grad", "ndsp_mlp_train",
-           "ndsp_batch_create", "ndsp_batch_destroy", "ndsp_batch_step", "ndsp_last_error")
+           "ndsp_batch_create", "ndsp_batch_destroy", "ndsp_batch_step", "ndsp_autotune",
+           "ndsp_copy_probe",
+           "ndsp_last_error")
 MLP_NPARAM = 1313
 
 
@@ -83,6 +85,8 @@ def load_library(path: str | None = None):
     L.ndsp_neighbours.argtypes = [vp, ctypes.c_int32, ip, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
     L.ndsp_set_timing.argtypes = [vp, ctypes.c_int32]
     L.ndsp_set_graph.argtypes = [vp, ctypes.c_int32]
+    L.ndsp_autotune.argtypes = [vp, ctypes.c_int32, ip]
+    L.ndsp_copy_probe.argtypes = [vp, ctypes.c_int64]
     L.ndsp_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.POINTER(vp)]
     L.ndsp_batch_destroy.argtypes = [vp]
     L.ndsp_batch_step.argtypes = [vp, ctypes.POINTER(_Stats), ctypes.POINTER(_Dt)]
@@ -243,6 +247,17 @@ class Simulation:
     def set_timing(self, on: bool):
         self._check(load_library().ndsp_set_timing(self.h, int(on)))
 
+    def autotune(self, reps: int = 2) -> dict:
+        """choose the tile shapes by timing derivs (ndsp_autotune); the state
+        is left unchanged"""
+        ch = np.zeros(3, dtype=np.int32)
+        self._check(load_library().ndsp_autotune(self.h, int(reps), _ip(ch)))
+        return {"force_workgroup": int(ch[0]), "density_workgroup": int(ch[1]), "cells_per_radius": int(ch[2])}
+
+    def copy_probe(self, nbytes: int):
+        """plain device copy of nbytes (counter calibration, ndsp_copy_probe)"""
+        self._check(load_library().ndsp_copy_probe(self.h, int(nbytes)))
+
     def set_graph(self, on: bool):
         """replay each step as two recorded HIP graphs (ndsp_set_graph)"""
         self._check(load_library().ndsp_set_graph(self.h, int(on)))
@@ -282,6 +297,10 @@ class Batch:
         self._check(load_library().ndsp_batch_step(self.h, self._st, self._dt))
         for k, sim in enumerate(self.sims):
             sim._stats(self._st[k], self._dt[k])
+
+    def autotune(self, reps: int = 2):
+        """tile shapes of every member (Simulation.autotune)"""
+        return [s.autotune(reps) for s in self.sims]
 
     def close(self):
         if getattr(self, "h", None):

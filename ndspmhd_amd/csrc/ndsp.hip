@@ -154,7 +154,7 @@ __global__ __launch_bounds__(BS) void k_binstats_final(DScal* sc, const double* 
 }
 
 // Cell grid of this binning, on the device (no host round trip): cells of
-// side >= radkern*hmax/NDSP_CELL_SUB (src/linkND.f90:72 uses radkern*hhmax);
+// side >= radkern*hmax/cell_sub (src/linkND.f90:72 uses radkern*hhmax);
 // walls tile their dimension exactly; cells are widened when the Morton key
 // space would exceed 2^cap_bits.  Also decides whether neighbour lists can
 // be used (2*radkern*hmax*(1+skin) below every periodic length, so the
@@ -166,7 +166,7 @@ __global__ void k_grid_setup(DScal* sc, Grid* gd, Grid box, Phys p, int cap_bits
   const double rmax = p.radkern * hmax;
   sc->grid_clamped = 0;
   if (!(rmax > 0.0) || !(rmax < DBL_MAX)) sc->err = NDSP_ERR_CELLS;
-  double cs = rmax / NDSP_CELL_SUB;
+  double cs = rmax / p.cell_sub;
   const int mbmax = cap_bits / p.ndim;
   for (int d = 0; d < 3; d++) {
     if (d >= p.ndim) {
@@ -935,8 +935,8 @@ __device__ __forceinline__ void nr_update(const Phys& p, double hh, double hin, 
   r.dhdrho = dhdrhoi;
 }
 
-template <int NDIM, typename PT, bool DUST>
-__global__ __launch_bounds__(BS) void k_nr_pass(KArgs a, Phys p, const Grid* __restrict__ gp, int* __restrict__ list_a,
+template <int NDIM, typename PT, bool DUST, int TB>
+__global__ __launch_bounds__(TB) void k_nr_pass(KArgs a, Phys p, const Grid* __restrict__ gp, int* __restrict__ list_a,
                                                 int* __restrict__ list_b) {
   // pass parameters from the device-side control (k_pass_ctl): pass its0
   // reads the active list written by the previous pass (list_a after odd
@@ -1227,9 +1227,10 @@ __global__ void k_pass_ctl(DScal* sc, int n, int reflect, int maxits) {
 // force: fused pair sums of get_rates
 // ===========================================================================
 
-// PHYS: 0 hydro, 1 MHD (imhd 11), 2 one-fluid dust (idust 4)
-template <int NDIM, typename PT, int PHYS>
-__global__ __launch_bounds__(BS) void k_force(KArgs a, Phys p, const Grid* __restrict__ gp) {
+// PHYS: 0 hydro, 1 MHD (imhd 11), 2 one-fluid dust (idust 4); TB threads
+// per workgroup (chosen by the autotuner)
+template <int NDIM, typename PT, int PHYS, int TB>
+__global__ __launch_bounds__(TB) void k_force(KArgs a, Phys p, const Grid* __restrict__ gp) {
   __shared__ PT tg[NDSP_IKERN + 1];
   {
     const PT* src = sizeof(PT) == 4 ? (const PT*)a.tabf : (const PT*)a.tabd;
@@ -1803,6 +1804,9 @@ __global__ void k_gather_d(const double* __restrict__ in, double* __restrict__ o
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) out[k] = in[sidx[k]];
 }
+__global__ void k_fill_d(double* __restrict__ a, int n, double v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = v;
+}
 __global__ void k_copy_d(const double* __restrict__ in, double* __restrict__ out, int n) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) out[k] = in[k];
@@ -1961,6 +1965,7 @@ static double nexp_of(const Phys& p) {
 }
 
 struct ndsp_ctx {
+  int tb_force = 256, tb_nr = 256;  // workgroup sizes of the pair kernels (autotuned)
   bool reflect = false;     // some wall reflects
   Grid* gdev = nullptr;     // cell grid of the last binning (device)
   int cap_bits = 0;         // Morton key-space capacity of cstart/cfill
@@ -2079,7 +2084,7 @@ static int check_err(ndsp_ctx* c, const DScal& h) {
 static int size_key_space(ndsp_ctx* c, const DScal& hs, int extra_bits) {
   const double rmax = c->p.radkern * hs.hmax_bin;
   if (!(rmax > 0.0) || !std::isfinite(rmax)) return set_err(NDSP_ERR_CELLS, "link: max h <= 0");
-  const double cs = rmax / NDSP_CELL_SUB;
+  const double cs = rmax / c->p.cell_sub;
   int mx = 1;
   for (int d = 0; d < c->p.ndim; d++) {
     const double ext = c->ibound[d] >= 2 ? c->xmax[d] - c->xmin[d] : hs.bbox_hi[d] - hs.bbox_lo[d];
@@ -2176,13 +2181,16 @@ static void launch_passes(ndsp_ctx* c, int K) {
     dispatch_ndim(c->p.ndim, [&](auto nd) {
       constexpr int ND = decltype(nd)::value;
       const bool fp64 = c->prec == NDSP_PREC_FP64, dust = c->p.idust != 0;
-      auto kern = fp64 ? (dust ? k_nr_pass<ND, double, true> : k_nr_pass<ND, double, false>)
-                       : (dust ? k_nr_pass<ND, float, true> : k_nr_pass<ND, float, false>);
-      hipLaunchKernelGGL(kern, dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p, (const Grid*)c->gdev, c->list_a,
-                         c->list_b);
+      const bool small = c->tb_nr == 64;
+      auto kern = fp64 ? (dust ? (small ? k_nr_pass<ND, double, true, 64> : k_nr_pass<ND, double, true, 256>)
+                               : (small ? k_nr_pass<ND, double, false, 64> : k_nr_pass<ND, double, false, 256>))
+                       : (dust ? (small ? k_nr_pass<ND, float, true, 64> : k_nr_pass<ND, float, true, 256>)
+                               : (small ? k_nr_pass<ND, float, false, 64> : k_nr_pass<ND, float, false, 256>));
+      hipLaunchKernelGGL(kern, dim3(nblk(n, c->tb_nr)), dim3(c->tb_nr), 0, c->st, a, c->p, (const Grid*)c->gdev,
+                         c->list_a, c->list_b);
       return 0;
     });
-    hipLaunchKernelGGL(k_red_final, dim3(1), dim3(BS), 0, c->st, (const double*)a.red, a.rcap, (int)nblk(n),
+    hipLaunchKernelGGL(k_red_final, dim3(1), dim3(BS), 0, c->st, (const double*)a.red, a.rcap, (int)nblk(n, c->tb_nr),
                        red_spec({RED_Q(RED_MAX, RED_F64, stressmax), RED_Q(RED_MAX, RED_F64, hmax_nr),
                                  RED_Q(RED_MAX, RED_I32, itsmax), RED_Q(RED_MIN, RED_I32, nmin),
                                  RED_Q(RED_MAX, RED_I32, nmax), RED_Q(RED_SUM, RED_U64, pairs_density),
@@ -2239,14 +2247,18 @@ static int do_rates(ndsp_ctx* c) {
       constexpr int ND = decltype(nd)::value;
       const int phys = c->p.imhd ? 1 : c->p.idust ? 2 : 0;
       void (*kern)(KArgs, Phys, const Grid* __restrict__);
-      if (c->prec == NDSP_PREC_FP64)
-        kern = phys == 1 ? k_force<ND, double, 1> : phys == 2 ? k_force<ND, double, 2> : k_force<ND, double, 0>;
-      else
-        kern = phys == 1 ? k_force<ND, float, 1> : phys == 2 ? k_force<ND, float, 2> : k_force<ND, float, 0>;
-      hipLaunchKernelGGL(kern, dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p, (const Grid*)c->gdev);
+      const bool small = c->tb_force == 64;
+      if (c->prec == NDSP_PREC_FP64) {
+        if (small) kern = phys == 1 ? k_force<ND, double, 1, 64> : phys == 2 ? k_force<ND, double, 2, 64> : k_force<ND, double, 0, 64>;
+        else kern = phys == 1 ? k_force<ND, double, 1, 256> : phys == 2 ? k_force<ND, double, 2, 256> : k_force<ND, double, 0, 256>;
+      } else {
+        if (small) kern = phys == 1 ? k_force<ND, float, 1, 64> : phys == 2 ? k_force<ND, float, 2, 64> : k_force<ND, float, 0, 64>;
+        else kern = phys == 1 ? k_force<ND, float, 1, 256> : phys == 2 ? k_force<ND, float, 2, 256> : k_force<ND, float, 0, 256>;
+      }
+      hipLaunchKernelGGL(kern, dim3(nblk(n, c->tb_force)), dim3(c->tb_force), 0, c->st, a, c->p, (const Grid*)c->gdev);
       return 0;
     });
-    hipLaunchKernelGGL(k_red_final, dim3(1), dim3(BS), 0, c->st, (const double*)a.red, a.rcap, (int)nblk(n),
+    hipLaunchKernelGGL(k_red_final, dim3(1), dim3(BS), 0, c->st, (const double*)a.red, a.rcap, (int)nblk(n, c->tb_force),
                        red_spec({RED_Q(RED_MAX, RED_F64, vsigmax), RED_Q(RED_MIN, RED_F64, dtcourant),
                                  RED_Q(RED_SUM, RED_U64, pairs_force)}),
                        a.sc, (const int*)nullptr);
@@ -2370,6 +2382,7 @@ int ndsp_create(const int32_t* iopt, const double* dopt, int64_t npart, int32_t 
   p.use_list = (nrefl == 0 && npart < (int64_t(1) << 25)) ? 1 : 0;
   if (iopt[17] != 0) p.use_list = 0;
   p.skin = 0.04;
+  p.cell_sub = 2;
   if (!bad.empty()) return set_err(NDSP_ERR_BAD_OPTION, bad);
   p.dndim = 1.0 / p.ndim;
   p.boxmin = DBL_MAX;
@@ -2432,7 +2445,7 @@ int ndsp_create(const int32_t* iopt, const double* dopt, int64_t npart, int32_t 
   ok &= hipMalloc(&a.G4, v4) == hipSuccess;
   ok &= hipMalloc(&a.G5, v1) == hipSuccess;
   ok &= hipMalloc((void**)&a.sc, sizeof(DScal)) == hipSuccess;
-  a.rcap = (int)nblk(n);
+  a.rcap = (int)nblk(n, 64);
   ok &= hipMalloc((void**)&a.red, sizeof(double) * RED_MAXQ * (size_t)a.rcap) == hipSuccess;
   {
     // list capacity: 1.8x the neighbour count of a uniform lattice within
@@ -2884,6 +2897,110 @@ int ndsp_batch_step(ndsp_batch* b, ndsp_stats* stats, ndsp_dt* dt) {
     if (e && !first_err) first_err = e;
   }
   return first_err;
+}
+
+
+// Autotuner of the tile shapes: workgroup size of the force and h-rho
+// kernels (64 or 256 threads: small simulations need more workgroups than
+// large ones) and cells per search radius of the binning (1 or 2).  Each
+// combination is timed over `reps` derivs evaluations (HIP events on the
+// context's stream); the fastest is kept.  The evolved state is saved before
+// and restored after the trials, so tuning leaves results unchanged.
+int ndsp_autotune(ndsp_ctx* c, int32_t reps, int32_t* choice) {
+  if (!c) return set_err(NDSP_ERR_BAD_ARG, "null context");
+  if (reps < 1) reps = 1;
+  const int n = c->n;
+  const bool timing = c->timing;
+  c->timing = false;
+  // save the evolved state and the scalars
+  double* save = nullptr;
+  if (hipMalloc(&save, sizeof(double) * (size_t)n * (F_NPERSIST + 1)) != hipSuccess)
+    return set_err(NDSP_ERR_ALLOC, "autotune: state copy");
+  int* save_i = (int*)(save + (size_t)n * F_NPERSIST);  // numneigh, nits
+  for (int fi = 0; fi < F_NPERSIST; fi++)
+    HIPCHK(hipMemcpyAsync(save + (size_t)fi * n, c->a.f[fi], sizeof(double) * n, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(save_i, c->a.numneigh, sizeof(int) * n, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(save_i + n, c->a.nits, sizeof(int) * n, hipMemcpyDeviceToDevice, c->st));
+  DScal sc0;
+  if (int e = sync_scalars(c, sc0)) { hipFree(save); return e; }
+  const bool had_rates = c->have_rates;
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  const int tbs[2] = {64, 256}, subs[2] = {1, 2};
+  double best = 1e300;
+  int bf = c->tb_force, bn = c->tb_nr, bc = c->p.cell_sub;
+  int err = 0;
+  for (int a = 0; a < 2 && !err; a++)
+    for (int b = 0; b < 2 && !err; b++)
+      for (int q = 0; q < 2 && !err; q++) {
+        c->tb_force = tbs[a];
+        c->tb_nr = tbs[b];
+        c->p.cell_sub = subs[q];
+        c->cap_bits = 0;  // the warm-up evaluation sizes the key space for this cell size
+        err = ndsp_derivs(c, nullptr, nullptr);
+        double tmin = 1e300;
+        for (int r = 0; r < reps && !err; r++) {
+          hipEventRecord(e0, c->st);
+          err = ndsp_derivs(c, nullptr, nullptr);
+          hipEventRecord(e1, c->st);
+          hipEventSynchronize(e1);
+          float ms = 0.f;
+          hipEventElapsedTime(&ms, e0, e1);
+          tmin = std::min(tmin, (double)ms);
+        }
+        if (!err && tmin < best) {
+          best = tmin;
+          bf = tbs[a];
+          bn = tbs[b];
+          bc = subs[q];
+        }
+      }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  c->tb_force = bf;
+  c->tb_nr = bn;
+  c->p.cell_sub = bc;
+  c->cap_bits = 0;  // the key space follows the chosen cell size at the next binning
+  for (int fi = 0; fi < F_NPERSIST; fi++)
+    hipMemcpyAsync(c->a.f[fi], save + (size_t)fi * n, sizeof(double) * n, hipMemcpyDeviceToDevice, c->st);
+  hipMemcpyAsync(c->a.numneigh, save_i, sizeof(int) * n, hipMemcpyDeviceToDevice, c->st);
+  hipMemcpyAsync(c->a.nits, save_i + n, sizeof(int) * n, hipMemcpyDeviceToDevice, c->st);
+  hipMemcpyAsync(c->a.sc, &sc0, sizeof(DScal), hipMemcpyHostToDevice, c->st);
+  hipStreamSynchronize(c->st);
+  hipFree(save);
+  c->have_rates = had_rates;
+  c->grid_valid = false;
+  c->graph_valid = false;
+  c->timing = timing;
+  if (err) return err;
+  if (choice) {
+    choice[0] = bf;
+    choice[1] = bn;
+    choice[2] = bc;
+  }
+  return 0;
+}
+
+// Copy of nbytes between two fresh device buffers with the library's plain
+// copy kernel: a reference of known traffic (nbytes read, nbytes written)
+// for calibrating hardware memory counters.
+int ndsp_copy_probe(ndsp_ctx* c, int64_t nbytes) {
+  if (!c || nbytes < 8) return set_err(NDSP_ERR_BAD_ARG, "copy_probe: bad arguments");
+  const long long nd = nbytes / 8;
+  if (nd > INT_MAX) return set_err(NDSP_ERR_BAD_ARG, "copy_probe: at most 2^31 doubles");
+  double *src = nullptr, *dst = nullptr;
+  if (hipMalloc(&src, sizeof(double) * nd) != hipSuccess || hipMalloc(&dst, sizeof(double) * nd) != hipSuccess) {
+    if (src) hipFree(src);
+    return set_err(NDSP_ERR_ALLOC, "copy_probe: buffers");
+  }
+  hipLaunchKernelGGL(k_fill_d, dim3(4096), dim3(BS), 0, c->st, src, (int)nd, 1.0);
+  HIPCHK(hipStreamSynchronize(c->st));
+  hipLaunchKernelGGL(k_copy_d, dim3(nblk(nd)), dim3(BS), 0, c->st, (const double*)src, dst, (int)nd);
+  HIPCHK(hipStreamSynchronize(c->st));
+  hipFree(src);
+  hipFree(dst);
+  return 0;
 }
 
 int ndsp_set_graph(ndsp_ctx* c, int32_t on) {

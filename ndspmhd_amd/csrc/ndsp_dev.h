@@ -5,11 +5,9 @@
 #include <stdint.h>
 
 #define NDSP_IKERN 4000
-// cells per search radius radkern*hmax along each dimension
-#define NDSP_CELL_SUB 2
 
 // Cell grid.  Dimensions with walls (periodic or reflecting) are tiled exactly
-// by nc cells of width w = (xmax - xmin)/nc >= radkern*hmax/NDSP_CELL_SUB, so
+// by nc cells of width w = (xmax - xmin)/nc >= radkern*hmax/cell_sub, so
 // a periodic or mirror image of a cell is again a whole cell; open dimensions
 // tile the particle bounding box.
 struct Grid {

@@ -36,6 +36,7 @@ struct Phys {
   double boxmin;  // smallest wall-to-wall length (search radii must stay below it)
   int use_list;   // neighbour lists (no reflecting walls)
   double skin;    // relative h margin of the lists
+  int cell_sub;   // cells per search radius radkern*hmax (autotuned)
 };
 
 // device scalars (one per context); doubles touched by atomics hold

@@ -63,3 +63,31 @@ def test_batch_member_against_oracle():
                 bad.append((m.opts.ndim, f, err))
     assert not bad, bad
     batch.close()
+
+
+def test_autotune_keeps_state_and_parity():
+    """the autotuner restores the evolved state bit for bit; the tuned
+    simulation still matches the oracle"""
+    import ndspmhd_amd as nd
+    import oracle
+    from ndspmhd_amd.setups import brio_wu_1d
+    o, st = brio_wu_1d(600)
+    sim = nd.Simulation(o, st, precision=1)
+    orc = oracle.Oracle(o, st)
+    sim.derivs()
+    orc.derivs()
+    before = {f: sim.get(f).copy() for f in FIELDS}
+    choice = sim.autotune(reps=1)
+    print("tile shapes", choice, flush=True)
+    assert choice["force_workgroup"] in (64, 256) and choice["density_workgroup"] in (64, 256)
+    assert choice["cells_per_radius"] in (1, 2)
+    for f in FIELDS:
+        assert np.array_equal(before[f], sim.get(f)), f
+    for _ in range(5):
+        sim.set_dt(orc.scalar("dt"), orc.scalar("time"))
+        orc.step()
+        sim.step()
+    for f in ("x", "vel", "Bevol", "en", "hh", "rho", "force"):
+        a, b = sim.get(f), orc.get(f)
+        err = np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+        assert err <= (1e-9 if f != "x" else 1e-11), (f, err)
