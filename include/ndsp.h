@@ -95,7 +95,8 @@ int ndsp_upload(ndsp_ctx *ctx, const double *x, const double *vel,
  * arrays: x vel pmass hh en Bevol alpha psi dustevol rho gradh gradgradh
  * drhodt dhdt dens pr spsound uu Bfield force dudt dendt dBevoldt daldt
  * dpsidt divB curlB gradpsi dustfrac rhogas rhodust ddustevoldt deltav
- * numneigh nits.  *width receives 1, 3 or ndim. */
+ * numneigh nits, and slot (the particle's current position in device
+ * memory, which periodic reorders change).  *width receives 1, 3 or ndim. */
 int ndsp_download(ndsp_ctx *ctx, const char *name, double *out, int32_t *width);
 
 /* Replaces set_linklist (src/linkND.f90:45) together with the ghost set-up

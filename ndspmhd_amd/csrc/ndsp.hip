@@ -63,9 +63,9 @@ static const int BS = 256;
 // ===========================================================================
 
 // boundary, src/boundaryND.f90:66-160
-__global__ void k_boundary(KArgs a, Phys p, Grid g, int in_step) {
+__device__ __forceinline__ void b_boundary(KArgs a, Phys p, Grid g, int in_step, VL vl) {
 #pragma clang fp contract(off)
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = vl.b * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   double* xs[3] = {a.f[F_X], a.f[F_Y], a.f[F_Z]};
   double* vs[3] = {a.f[F_VX], a.f[F_VY], a.f[F_VZ]};
@@ -93,14 +93,17 @@ __global__ void k_boundary(KArgs a, Phys p, Grid g, int in_step) {
     }
   }
 }
+__global__ void k_boundary(KArgs a, Phys p, Grid g, int in_step) {
+  b_boundary(a, p, g, in_step, VL{(int)blockIdx.x, (int)gridDim.x});
+}
 
 // per-block max h and bounding box -> part[blk*8 + 0..6]
-__global__ void k_binstats(KArgs a, int ndim, double* part) {
+__device__ __forceinline__ void b_binstats(KArgs a, int ndim, double* part, VL vl) {
   __shared__ double sm[7][BS];
   int t = threadIdx.x;
   double hm = 0.0, lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
   const double* xs[3] = {a.f[F_X], a.f[F_Y], a.f[F_Z]};
-  for (int i = blockIdx.x * blockDim.x + t; i < a.n; i += gridDim.x * blockDim.x) {
+  for (int i = vl.b * blockDim.x + t; i < a.n; i += vl.nb * blockDim.x) {
     hm = fmax(hm, a.f[F_H][i]);
     for (int d = 0; d < ndim; d++) {
       lo[d] = fmin(lo[d], xs[d][i]);
@@ -120,11 +123,14 @@ __global__ void k_binstats(KArgs a, int ndim, double* part) {
     }
     __syncthreads();
   }
-  if (t < 7) part[blockIdx.x * 8 + t] = sm[t][0];
+  if (t < 7) part[vl.b * 8 + t] = sm[t][0];
+}
+__global__ void k_binstats(KArgs a, int ndim, double* part) {
+  b_binstats(a, ndim, part, VL{(int)blockIdx.x, (int)gridDim.x});
 }
 
 // one block of BS threads folds the per-block partials of k_binstats
-__global__ __launch_bounds__(BS) void k_binstats_final(DScal* sc, const double* part, int nb) {
+__device__ __forceinline__ void b_binstats_final(DScal* sc, const double* part, int nb, VL vl) {
   __shared__ double sm[7][BS];
   double v[7] = {0.0, DBL_MAX, DBL_MAX, DBL_MAX, -DBL_MAX, -DBL_MAX, -DBL_MAX};
   for (int b = threadIdx.x; b < nb; b += blockDim.x) {
@@ -152,6 +158,9 @@ __global__ __launch_bounds__(BS) void k_binstats_final(DScal* sc, const double* 
     for (int d = 0; d < 3; d++) { sc->bbox_lo[d] = sm[1 + d][0]; sc->bbox_hi[d] = sm[4 + d][0]; }
   }
 }
+__global__ __launch_bounds__(BS) void k_binstats_final(DScal* sc, const double* part, int nb) {
+  b_binstats_final(sc, part, nb, VL{(int)blockIdx.x, (int)gridDim.x});
+}
 
 // Cell grid of this binning, on the device (no host round trip): cells of
 // side >= radkern*hmax/cell_sub (src/linkND.f90:72 uses radkern*hhmax);
@@ -159,8 +168,8 @@ __global__ __launch_bounds__(BS) void k_binstats_final(DScal* sc, const double* 
 // space would exceed 2^cap_bits.  Also decides whether neighbour lists can
 // be used (2*radkern*hmax*(1+skin) below every periodic length, so the
 // minimum image of a neighbour is unique).
-__global__ void k_grid_setup(DScal* sc, Grid* gd, Grid box, Phys p, int cap_bits) {
-  if (threadIdx.x | blockIdx.x) return;
+__device__ __forceinline__ void b_grid_setup(DScal* sc, Grid* gd, Grid box, Phys p, int cap_bits, VL vl) {
+  if (threadIdx.x | vl.b) return;
   Grid g = box;
   const double hmax = sc->hmax_bin;
   const double rmax = p.radkern * hmax;
@@ -210,15 +219,24 @@ __global__ void k_grid_setup(DScal* sc, Grid* gd, Grid box, Phys p, int cap_bits
   sc->list_overflow = 0;
   *gd = g;
 }
-
-__global__ void k_fill_cells(int* __restrict__ a, const Grid* __restrict__ gd) {
-  const long long m = gd->ncell + 1;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (long long)gridDim.x * blockDim.x)
-    a[i] = 0;
+__global__ void k_grid_setup(DScal* sc, Grid* gd, Grid box, Phys p, int cap_bits) {
+  b_grid_setup(sc, gd, box, p, cap_bits, VL{(int)blockIdx.x, (int)gridDim.x});
 }
 
+__device__ __forceinline__ void b_fill_cells(int* __restrict__ a, const Grid* __restrict__ gd, VL vl) {
+  const long long m = gd->ncell + 1;
+  for (long long i = (long long)vl.b * blockDim.x + threadIdx.x; i < m; i += (long long)vl.nb * blockDim.x)
+    a[i] = 0;
+}
+__global__ void k_fill_cells(int* __restrict__ a, const Grid* __restrict__ gd) {
+  b_fill_cells(a, gd, VL{(int)blockIdx.x, (int)gridDim.x});
+}
+
+__device__ __forceinline__ void b_fill_i(int* __restrict__ a, int n, int v, VL vl) {
+  for (int i = vl.b * blockDim.x + threadIdx.x; i < n; i += vl.nb * blockDim.x) a[i] = v;
+}
 __global__ void k_fill_i(int* __restrict__ a, int n, int v) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = v;
+  b_fill_i(a, n, v, VL{(int)blockIdx.x, (int)gridDim.x});
 }
 
 __device__ __forceinline__ int cell_coord(const Grid& g, int d, double x) {
@@ -228,8 +246,8 @@ __device__ __forceinline__ int cell_coord(const Grid& g, int d, double x) {
   return c;
 }
 
-__global__ void k_cellid(KArgs a, const Grid* __restrict__ gp) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void b_cellid(KArgs a, const Grid* __restrict__ gp, VL vl) {
+  int i = vl.b * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const Grid g = *gp;
   int cx = cell_coord(g, 0, a.f[F_X][i]);
@@ -239,14 +257,17 @@ __global__ void k_cellid(KArgs a, const Grid* __restrict__ gp) {
   a.cellid[i] = c;
   atomicAdd(&a.cfill[c], 1);
 }
+__global__ void k_cellid(KArgs a, const Grid* __restrict__ gp) {
+  b_cellid(a, gp, VL{(int)blockIdx.x, (int)gridDim.x});
+}
 
 // exclusive scan, 3 phases (1024 elements per block)
-__global__ void k_scan_local(const int* in, int* out, int* bsum, const Grid* __restrict__ gp) {
+__device__ __forceinline__ void b_scan_local(const int* in, int* out, int* bsum, const Grid* __restrict__ gp, VL vl) {
   __shared__ int sm[BS];
   const long long n = gp->ncell + 1;
-  if ((long long)blockIdx.x * BS * 4 >= n) return;
+  if ((long long)vl.b * BS * 4 >= n) return;
   const int ITEMS = 4;
-  long long base = (long long)blockIdx.x * BS * ITEMS + (long long)threadIdx.x * ITEMS;
+  long long base = (long long)vl.b * BS * ITEMS + (long long)threadIdx.x * ITEMS;
   int v[ITEMS];
   int tot = 0;
   for (int k = 0; k < ITEMS; k++) {
@@ -268,10 +289,13 @@ __global__ void k_scan_local(const int* in, int* out, int* bsum, const Grid* __r
     if (idx < n) out[idx] = run;
     run += v[k];
   }
-  if (threadIdx.x == BS - 1) bsum[blockIdx.x] = sm[BS - 1];
+  if (threadIdx.x == BS - 1) bsum[vl.b] = sm[BS - 1];
+}
+__global__ void k_scan_local(const int* in, int* out, int* bsum, const Grid* __restrict__ gp) {
+  b_scan_local(in, out, bsum, gp, VL{(int)blockIdx.x, (int)gridDim.x});
 }
 
-__global__ void k_scan_bsum(int* bsum, const Grid* __restrict__ gp, int* total_out) {
+__device__ __forceinline__ void b_scan_bsum(int* bsum, const Grid* __restrict__ gp, int* total_out, VL vl) {
   __shared__ int sm[BS];
   const int nb = (int)((gp->ncell + 1 + BS * 4 - 1) / (BS * 4));
   __shared__ int carry;
@@ -295,25 +319,34 @@ __global__ void k_scan_bsum(int* bsum, const Grid* __restrict__ gp, int* total_o
   }
   if (threadIdx.x == 0 && total_out) *total_out = carry;
 }
-
-__global__ void k_scan_add(int* out, const int* bsum, const Grid* __restrict__ gp) {
-  const long long n = gp->ncell + 1;
-  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx < n) out[idx] += bsum[idx / (BS * 4)];
+__global__ void k_scan_bsum(int* bsum, const Grid* __restrict__ gp, int* total_out) {
+  b_scan_bsum(bsum, gp, total_out, VL{(int)blockIdx.x, (int)gridDim.x});
 }
 
-__global__ void k_scatter(KArgs a) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void b_scan_add(int* out, const int* bsum, const Grid* __restrict__ gp, VL vl) {
+  const long long n = gp->ncell + 1;
+  long long idx = (long long)vl.b * blockDim.x + threadIdx.x;
+  if (idx < n) out[idx] += bsum[idx / (BS * 4)];
+}
+__global__ void k_scan_add(int* out, const int* bsum, const Grid* __restrict__ gp) {
+  b_scan_add(out, bsum, gp, VL{(int)blockIdx.x, (int)gridDim.x});
+}
+
+__device__ __forceinline__ void b_scatter(KArgs a, VL vl) {
+  int i = vl.b * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   int c = a.cellid[i];
   int pos = a.cstart[c] + atomicAdd(&a.cfill[c], 1);
   a.sidx[pos] = i;
 }
+__global__ void k_scatter(KArgs a) {
+  b_scatter(a, VL{(int)blockIdx.x, (int)gridDim.x});
+}
 
 // make the order inside each cell ascending in slot index (deterministic,
 // and a stable sort by cell key overall)
-__global__ void k_cellsort(KArgs a, const Grid* __restrict__ gp) {
-  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void b_cellsort(KArgs a, const Grid* __restrict__ gp, VL vl) {
+  long long c = (long long)vl.b * blockDim.x + threadIdx.x;
   if (c >= gp->ncell) return;
   int b = a.cstart[c], e = a.cstart[c + 1];
   for (int k = b + 1; k < e; k++) {
@@ -326,14 +359,17 @@ __global__ void k_cellsort(KArgs a, const Grid* __restrict__ gp) {
     a.sidx[j + 1] = v;
   }
 }
+__global__ void k_cellsort(KArgs a, const Grid* __restrict__ gp) {
+  b_cellsort(a, gp, VL{(int)blockIdx.x, (int)gridDim.x});
+}
 
 // ===========================================================================
 // neighbour views
 // ===========================================================================
 
 template <typename PT>
-__global__ void k_pack_density(KArgs a, Phys p) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void b_pack_density(KArgs a, Phys p, VL vl) {
+  int k = vl.b * blockDim.x + threadIdx.x;
   if (k >= a.n) return;
   int i = a.sidx[k];
   double h = a.f[F_H][i];
@@ -345,10 +381,14 @@ __global__ void k_pack_density(KArgs a, Phys p) {
   // (dustfrac, src/density_sums.f90:276-278), not the predicted dustevol
   if (p.idust) ((PT*)a.De)[k] = (PT)a.f[F_DUSTFRAC][i];
 }
+template <typename PT>
+__global__ void k_pack_density(KArgs a, Phys p) {
+  b_pack_density<PT>(a, p, VL{(int)blockIdx.x, (int)gridDim.x});
+}
 
 template <typename PT>
-__global__ void k_pack_force(KArgs a, Phys p) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void b_pack_force(KArgs a, Phys p, VL vl) {
+  int k = vl.b * blockDim.x + threadIdx.x;
   if (k >= a.n) return;
   int i = a.sidx[k];
   double h = a.f[F_H][i];
@@ -361,6 +401,10 @@ __global__ void k_pack_force(KArgs a, Phys p) {
     ((T4*)a.G3)[k] = T4{(PT)a.f[F_BX][i], (PT)a.f[F_BY][i], (PT)a.f[F_BZ][i], (PT)a.f[F_PSI][i]};
   ((T4*)a.G4)[k] = T4{(PT)h, (PT)a.f[F_DENS][i], (PT)a.f[F_DUSTFRAC][i], (PT)a.f[F_RHOGAS][i]};
   if (p.idust) ((PT*)a.G5)[k] = (PT)a.f[F_RHODUST][i];
+}
+template <typename PT>
+__global__ void k_pack_force(KArgs a, Phys p) {
+  b_pack_force<PT>(a, p, VL{(int)blockIdx.x, (int)gridDim.x});
 }
 
 // ===========================================================================
@@ -558,7 +602,7 @@ __device__ __forceinline__ int wave_max_i(int v) {
 // (k_nblist_sort moves them into place).
 #define NBL_CELLS 256
 template <int NDIM>
-__global__ __launch_bounds__(64) void k_nblist(KArgs a, Phys p, const Grid* __restrict__ gp) {
+__device__ __forceinline__ void b_nblist(KArgs a, Phys p, const Grid* __restrict__ gp, VL vl) {
   __shared__ double4 sraw[128];
   __shared__ float4 srel[128];
   __shared__ unsigned se[128];
@@ -567,7 +611,7 @@ __global__ __launch_bounds__(64) void k_nblist(KArgs a, Phys p, const Grid* __re
   __shared__ int cbeg[NBL_CELLS];
   __shared__ int cpre[NBL_CELLS + 1];
   const int lane = threadIdx.x;
-  const int k = xcd_tile() * 64 + lane;
+  const int k = xcd_tile(vl) * 64 + lane;
   const bool act = k < a.n;
   if (!a.sc->list_on) {
     if (act) a.nbc[k] = -1;
@@ -777,12 +821,16 @@ __global__ __launch_bounds__(64) void k_nblist(KArgs a, Phys p, const Grid* __re
     a.hcap[k] = s1 / pk.w;
   }
 }
+template <int NDIM>
+__global__ __launch_bounds__(64) void k_nblist(KArgs a, Phys p, const Grid* __restrict__ gp) {
+  b_nblist<NDIM>(a, p, gp, VL{(int)blockIdx.x, (int)gridDim.x});
+}
 
 // extras (written from the end of the row) into [nbc, nbc + nbx) in
 // ascending entry order, so sums are deterministic; a row that did not hold
 // the gather list and the extras falls back to the cell scan
-__global__ __launch_bounds__(BS) void k_nblist_sort(KArgs a) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void b_nblist_sort(KArgs a, VL vl) {
+  const int k = vl.b * blockDim.x + threadIdx.x;
   if (k >= a.n || !a.sc->list_on) return;
   const int c0 = a.nbc[k], nx = a.nbx[k];
   if (c0 < 0 || c0 + nx > a.maxnb) {
@@ -806,6 +854,9 @@ __global__ __launch_bounds__(BS) void k_nblist_sort(KArgs a) {
   if (D != E)
     for (int m = 0; m < nx; m++) D[m] = E[m];
 }
+__global__ __launch_bounds__(BS) void k_nblist_sort(KArgs a) {
+  b_nblist_sort(a, VL{(int)blockIdx.x, (int)gridDim.x});
+}
 
 // ===========================================================================
 // global reductions without contended atomics: every block reduces its
@@ -817,11 +868,11 @@ __device__ __forceinline__ double red_combine(int kind, double a, double b) {
   return kind == RED_SUM ? a + b : kind == RED_MAX ? fmax(a, b) : fmin(a, b);
 }
 
-// v[q] of every thread of the block -> red[q * rcap + blockIdx.x]
+// v[q] of every thread of the block -> red[q * rcap + (virtual block)]
 // (kinds are compile-time per call site; every thread must call this)
 template <int NQ>
 __device__ __forceinline__ void block_partials(double (&v)[NQ], const int (&kind)[NQ], double* red,
-                                               int rcap) {
+                                               int rcap, VL vl) {
   __shared__ double sred[NQ][BS / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
@@ -835,12 +886,12 @@ __device__ __forceinline__ void block_partials(double (&v)[NQ], const int (&kind
     const int q = threadIdx.x;
     double x = sred[q][0];
     for (int w = 1; w < (int)(blockDim.x >> 6); w++) x = red_combine(kind[q], x, sred[q][w]);
-    red[(size_t)q * rcap + blockIdx.x] = x;
+    red[(size_t)q * rcap + vl.b] = x;
   }
 }
 
-__global__ __launch_bounds__(BS) void k_red_final(const double* __restrict__ red, int rcap, int nblocks,
-                                                  RedSpec spec, DScal* sc, const int* skip) {
+__device__ __forceinline__ void b_red_final(const double* __restrict__ red, int rcap, int nblocks,
+                                                  RedSpec spec, DScal* sc, const int* skip, VL vl) {
   if (skip && *skip) return;
   __shared__ double sm[BS];
   for (int q = 0; q < spec.nq; q++) {
@@ -870,6 +921,10 @@ __global__ __launch_bounds__(BS) void k_red_final(const double* __restrict__ red
     }
     __syncthreads();
   }
+}
+__global__ __launch_bounds__(BS) void k_red_final(const double* __restrict__ red, int rcap, int nblocks,
+                                                  RedSpec spec, DScal* sc, const int* skip) {
+  b_red_final(red, rcap, nblocks, spec, sc, skip, VL{(int)blockIdx.x, (int)gridDim.x});
 }
 
 static RedSpec red_spec(std::initializer_list<std::pair<int, int>> q) {
@@ -936,8 +991,8 @@ __device__ __forceinline__ void nr_update(const Phys& p, double hh, double hin, 
 }
 
 template <int NDIM, typename PT, bool DUST, int TB>
-__global__ __launch_bounds__(TB) void k_nr_pass(KArgs a, Phys p, const Grid* __restrict__ gp, int* __restrict__ list_a,
-                                                int* __restrict__ list_b) {
+__device__ __forceinline__ void b_nr_pass(KArgs a, Phys p, const Grid* __restrict__ gp, int* __restrict__ list_a,
+                                                int* __restrict__ list_b, VL vl) {
   // pass parameters from the device-side control (k_pass_ctl): pass its0
   // reads the active list written by the previous pass (list_a after odd
   // passes, list_b after even ones) and appends its unconverged particles to
@@ -960,7 +1015,7 @@ __global__ __launch_bounds__(TB) void k_nr_pass(KArgs a, Phys p, const Grid* __r
   }
   __syncthreads();
   const Grid g = *gp;
-  const int tid = xcd_tile() * blockDim.x + threadIdx.x;
+  const int tid = xcd_tile(vl) * blockDim.x + threadIdx.x;
   const bool active = tid < nlist;
   bool unconverged = false;
   const int k = active ? (list ? list[tid] : tid) : 0;
@@ -1150,7 +1205,7 @@ __global__ __launch_bounds__(TB) void k_nr_pass(KArgs a, Phys p, const Grid* __r
   {
     double rv[7] = {r_stress, r_hmax, r_its, r_nmin, r_nmax, r_pairs, r_ncalc};
     const int rk[7] = {RED_MAX, RED_MAX, RED_MAX, RED_MIN, RED_MAX, RED_SUM, RED_SUM};
-    block_partials<7>(rv, rk, a.red, a.rcap);
+    block_partials<7>(rv, rk, a.red, a.rcap, vl);
   }
   // append unconverged particles to the next pass's list (one atomic per wave)
   if (!persistent) {
@@ -1163,26 +1218,34 @@ __global__ __launch_bounds__(TB) void k_nr_pass(KArgs a, Phys p, const Grid* __r
     if (unconverged) next_list[base + __popcll(m & ((1ull << lane) - 1ull))] = k;
   }
 }
+template <int NDIM, typename PT, bool DUST, int TB>
+__global__ __launch_bounds__(TB) void k_nr_pass(KArgs a, Phys p, const Grid* __restrict__ gp, int* __restrict__ list_a,
+                                                int* __restrict__ list_b) {
+  b_nr_pass<NDIM, PT, DUST, TB>(a, p, gp, list_a, list_b, VL{(int)blockIdx.x, (int)gridDim.x});
+}
 
 // P4.w = 1/h of every particle before a symmetric pass after the first, and
 // the reflecting ghost extents after a relink (as decided by k_pass_ctl)
-__global__ void k_refresh_hinv(KArgs a, Phys p) {
+__device__ __forceinline__ void b_refresh_hinv(KArgs a, Phys p, VL vl) {
   if (a.sc->pc_done || !a.sc->pc_refresh) return;
   const int refresh_dxb = a.sc->pc_relink;
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int k = vl.b * blockDim.x + threadIdx.x;
   if (k >= a.n) return;
   const double h = a.f[F_H][a.sidx[k]];
   a.P4[k].w = 1.0 / h;
   if (refresh_dxb) a.dxb[k] = p.radkern * h;
 }
+__global__ void k_refresh_hinv(KArgs a, Phys p) {
+  b_refresh_hinv(a, p, VL{(int)blockIdx.x, (int)gridDim.x});
+}
 
 // max h over all particles -> hmax_cur (and the ghost extent on a relink)
-__global__ void k_hmax(KArgs a, DScal* sc) {
+__device__ __forceinline__ void b_hmax(KArgs a, DScal* sc, VL vl) {
   if (sc->pc_done) return;
   const int relink = sc->pc_relink;
   __shared__ double sm[BS];
   double hm = 0.0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x)
+  for (int i = vl.b * blockDim.x + threadIdx.x; i < a.n; i += vl.nb * blockDim.x)
     hm = fmax(hm, a.f[F_H][i]);
   sm[threadIdx.x] = hm;
   __syncthreads();
@@ -1195,14 +1258,17 @@ __global__ void k_hmax(KArgs a, DScal* sc) {
     if (relink) atomic_max_pos(&sc->hhmax_ghost, sm[0]);
   }
 }
+__global__ void k_hmax(KArgs a, DScal* sc) {
+  b_hmax(a, sc, VL{(int)blockIdx.x, (int)gridDim.x});
+}
 
 // After a pass (src/iterate_density.f90:280-300): finished when the pass was
 // the persistent one, failed, or left no particle unconverged; otherwise the
 // unconverged ones form the next pass, symmetric if they are all particles
 // (:131-135), synchronous while walls reflect (a relink, :260-262, rebuilds
 // the ghosts with every particle's current h).
-__global__ void k_pass_ctl(DScal* sc, int n, int reflect, int maxits) {
-  if (threadIdx.x | blockIdx.x) return;
+__device__ __forceinline__ void b_pass_ctl(DScal* sc, int n, int reflect, int maxits, VL vl) {
+  if (threadIdx.x | vl.b) return;
   if (sc->pc_done) return;
   if (sc->pc_persistent || sc->err || sc->ncalc_next == 0 || sc->pc_its > maxits) {
     sc->pc_done = 1;
@@ -1221,6 +1287,9 @@ __global__ void k_pass_ctl(DScal* sc, int n, int reflect, int maxits) {
   sc->hmax_cur = 0.0;
   if (relink) sc->hhmax_ghost = 0.0;
 }
+__global__ void k_pass_ctl(DScal* sc, int n, int reflect, int maxits) {
+  b_pass_ctl(sc, n, reflect, maxits, VL{(int)blockIdx.x, (int)gridDim.x});
+}
 
 
 // ===========================================================================
@@ -1230,7 +1299,7 @@ __global__ void k_pass_ctl(DScal* sc, int n, int reflect, int maxits) {
 // PHYS: 0 hydro, 1 MHD (imhd 11), 2 one-fluid dust (idust 4); TB threads
 // per workgroup (chosen by the autotuner)
 template <int NDIM, typename PT, int PHYS, int TB>
-__global__ __launch_bounds__(TB) void k_force(KArgs a, Phys p, const Grid* __restrict__ gp) {
+__device__ __forceinline__ void b_force(KArgs a, Phys p, const Grid* __restrict__ gp, VL vl) {
   __shared__ PT tg[NDSP_IKERN + 1];
   {
     const PT* src = sizeof(PT) == 4 ? (const PT*)a.tabf : (const PT*)a.tabd;
@@ -1238,7 +1307,7 @@ __global__ __launch_bounds__(TB) void k_force(KArgs a, Phys p, const Grid* __res
   }
   __syncthreads();
   const Grid g = *gp;
-  const int k = xcd_tile() * blockDim.x + threadIdx.x;
+  const int k = xcd_tile(vl) * blockDim.x + threadIdx.x;
   const bool active = k < a.n;
   typedef typename V4<PT>::type T4;
   const T4* G0 = (const T4*)a.G0;
@@ -1504,14 +1573,18 @@ __global__ __launch_bounds__(TB) void k_force(KArgs a, Phys p, const Grid* __res
   {
     double rv[3] = {(double)vsigmax_l, dtc_l, (double)npairs};
     const int rk[3] = {RED_MAX, RED_MIN, RED_SUM};
-    block_partials<3>(rv, rk, a.red, a.rcap);
+    block_partials<3>(rv, rk, a.red, a.rcap, vl);
   }
+}
+template <int NDIM, typename PT, int PHYS, int TB>
+__global__ __launch_bounds__(TB) void k_force(KArgs a, Phys p, const Grid* __restrict__ gp) {
+  b_force<NDIM, PT, PHYS, TB>(a, p, gp, VL{(int)blockIdx.x, (int)gridDim.x});
 }
 
 // per-particle finalisation, src/ratesND_mhd.f90:482-943 (FP64)
-__global__ void k_finalize(KArgs a, Phys p) {
+__device__ __forceinline__ void b_finalize(KArgs a, Phys p, VL vl) {
 #pragma clang fp contract(off)
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = vl.b * blockDim.x + threadIdx.x;
   double fh = 0.0, dtd = DBL_MAX;
   if (i < a.n) {
     double** f = a.f;
@@ -1592,29 +1665,38 @@ __global__ void k_finalize(KArgs a, Phys p) {
   }
   double rv[2] = {fh, dtd};
   const int rk[2] = {RED_MAX, RED_MIN};
-  block_partials<2>(rv, rk, a.red, a.rcap);
+  block_partials<2>(rv, rk, a.red, a.rcap, vl);
+}
+__global__ void k_finalize(KArgs a, Phys p) {
+  b_finalize(a, p, VL{(int)blockIdx.x, (int)gridDim.x});
 }
 
 // timestep, src/ratesND_mhd.f90:938-943 and src/stepND_leapfrog_mhd.f90:252
-__global__ void k_dt(DScal* sc, Phys p) {
-  if (threadIdx.x | blockIdx.x) return;
+__device__ __forceinline__ void b_dt(DScal* sc, Phys p, VL vl) {
+  if (threadIdx.x | vl.b) return;
   double dtforce = DBL_MAX;
   if (sc->fhmax > 0.0) dtforce = fmin(dtforce, sqrt(1.0 / sc->fhmax));
   sc->dtforce = dtforce;
   sc->dt = fmin(fmin(fmin(p.C_force * dtforce, p.C_cour * sc->dtcourant), 0.9 * sc->dtdrag),
                 p.C_force * DBL_MAX);
 }
+__global__ void k_dt(DScal* sc, Phys p) {
+  b_dt(sc, p, VL{(int)blockIdx.x, (int)gridDim.x});
+}
 
 // end of a leapfrog step: advance the time by the step's dt and take the dt
 // chosen by this step's rates for the next one (src/evolve.f90:106-110)
-__global__ void k_step_end(DScal* sc) {
-  if (threadIdx.x | blockIdx.x) return;
+__device__ __forceinline__ void b_step_end(DScal* sc, VL vl) {
+  if (threadIdx.x | vl.b) return;
   sc->time = sc->time + sc->dt_cur;
   sc->dt_cur = sc->dt;
 }
+__global__ void k_step_end(DScal* sc) {
+  b_step_end(sc, VL{(int)blockIdx.x, (int)gridDim.x});
+}
 
-__global__ void k_init_density_scalars(DScal* sc, double bconstmax, int n) {
-  if (threadIdx.x | blockIdx.x) return;
+__device__ __forceinline__ void b_init_density_scalars(DScal* sc, double bconstmax, int n, VL vl) {
+  if (threadIdx.x | vl.b) return;
   sc->pc_nlist = n;
   sc->pc_its = 1;
   sc->pc_sym = 1;
@@ -1635,23 +1717,29 @@ __global__ void k_init_density_scalars(DScal* sc, double bconstmax, int n) {
   sc->pairs_density = 0;
   sc->ncalc_total = 0;
 }
+__global__ void k_init_density_scalars(DScal* sc, double bconstmax, int n) {
+  b_init_density_scalars(sc, bconstmax, n, VL{(int)blockIdx.x, (int)gridDim.x});
+}
 
-__global__ void k_init_force_scalars(DScal* sc) {
-  if (threadIdx.x | blockIdx.x) return;
+__device__ __forceinline__ void b_init_force_scalars(DScal* sc, VL vl) {
+  if (threadIdx.x | vl.b) return;
   sc->vsigmax = 0.0;
   sc->dtcourant = 1.0e6;
   sc->fhmax = 0.0;
   sc->dtdrag = DBL_MAX;
   sc->pairs_force = 0;
 }
+__global__ void k_init_force_scalars(DScal* sc) {
+  b_init_force_scalars(sc, VL{(int)blockIdx.x, (int)gridDim.x});
+}
 
 // ===========================================================================
 // leapfrog, src/stepND_leapfrog_mhd.f90:31-275
 // ===========================================================================
 
-__global__ void k_predict(KArgs a, Phys p) {
+__device__ __forceinline__ void b_predict(KArgs a, Phys p, VL vl) {
 #pragma clang fp contract(off)
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = vl.b * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const double dt = a.sc->dt_cur;
   double** f = a.f;
@@ -1687,10 +1775,13 @@ __global__ void k_predict(KArgs a, Phys p) {
   if (p.idivbzero >= 2) f[F_PSI][i] = f[F_PSIIN][i] + dt * f[F_DPSIDTIN][i];
   if (p.idust) f[F_EPS][i] = f[F_EPSIN][i] + dt * f[F_DEPSIN][i];
 }
+__global__ void k_predict(KArgs a, Phys p) {
+  b_predict(a, p, VL{(int)blockIdx.x, (int)gridDim.x});
+}
 
-__global__ void k_correct(KArgs a, Phys p) {
+__device__ __forceinline__ void b_correct(KArgs a, Phys p, VL vl) {
 #pragma clang fp contract(off)
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = vl.b * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const double dt = a.sc->dt_cur;
   double** f = a.f;
@@ -1709,6 +1800,9 @@ __global__ void k_correct(KArgs a, Phys p) {
     if (p.iavlim[c] != 0) f[as[c]][i] = fmin(f[F_A0IN + c][i] + hdt * (f[das[c]][i] + f[F_DA0IN + c][i]), 1.0);
   if (p.idivbzero >= 2) f[F_PSI][i] = f[F_PSIIN][i] + hdt * (f[F_DPSIDT][i] + f[F_DPSIDTIN][i]);
   if (p.idust) f[F_EPS][i] = f[F_EPSIN][i] + hdt * (f[F_DEPS][i] + f[F_DEPSIN][i]);
+}
+__global__ void k_correct(KArgs a, Phys p) {
+  b_correct(a, p, VL{(int)blockIdx.x, (int)gridDim.x});
 }
 
 // ===========================================================================
@@ -1903,6 +1997,102 @@ __global__ void k_mlp_target(KArgs a, float* __restrict__ X) {
 
 // kernel tables, src/kernelND.f90:127-161 (set-up), 211-269 (M6 quintic),
 // 4143-4190 (M4 cubic), 4279-4289 (normalisation)
+// ===========================================================================
+// launches over a batch: one launch per phase covers every member of a
+// group (same dimension, physics and precision).  Workgroup b belongs to the
+// member m with off[m] <= b < off[m+1] and plays block b - off[m] of the
+// member's own launch, so each member computes exactly what its own launch
+// would.
+// ===========================================================================
+
+struct MArgs {
+  KArgs a;
+  Phys p;
+  Grid* gdev;
+  Grid box;
+  double* part;   // binning partials
+  int* bsum;      // scan block sums
+  int* list_a;
+  int* list_b;
+  DScal* out;     // slot of this member in the batch's scalar array
+  double bconstmax;
+  int cap_bits;
+  int reflect;
+  int nbp;        // binning-statistics blocks
+  int nb_nr, nb_force, nb_256;  // blocks of the h-rho, force and elementwise launches
+};
+
+__device__ __forceinline__ int member_of(const int* __restrict__ off, int nm, int b) {
+  int lo = 0, hi = nm - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+#define NDSP_MEMBER                                        \
+  const int m_ = member_of(off, nm, (int)blockIdx.x);      \
+  const MArgs& M = ma[m_];                                 \
+  const VL vl{(int)blockIdx.x - off[m_], off[m_ + 1] - off[m_]};
+#define NDSP_MK(name) __global__ void name(const MArgs* __restrict__ ma, const int* __restrict__ off, int nm)
+
+NDSP_MK(km_predict) { NDSP_MEMBER b_predict(M.a, M.p, vl); }
+NDSP_MK(km_correct) { NDSP_MEMBER b_correct(M.a, M.p, vl); }
+NDSP_MK(km_boundary_step) { NDSP_MEMBER b_boundary(M.a, M.p, M.box, 1, vl); }
+NDSP_MK(km_err_reset) { NDSP_MEMBER if (threadIdx.x == 0) M.a.sc->err = 0; }
+NDSP_MK(km_binstats) { NDSP_MEMBER b_binstats(M.a, M.p.ndim, M.part, vl); }
+NDSP_MK(km_binstats_final) { NDSP_MEMBER b_binstats_final(M.a.sc, M.part, M.nbp, vl); }
+NDSP_MK(km_grid_setup) { NDSP_MEMBER b_grid_setup(M.a.sc, M.gdev, M.box, M.p, M.cap_bits, vl); }
+NDSP_MK(km_fill_cells) { NDSP_MEMBER b_fill_cells(M.a.cfill, M.gdev, vl); }
+NDSP_MK(km_cellid) { NDSP_MEMBER b_cellid(M.a, M.gdev, vl); }
+NDSP_MK(km_scan_local) { NDSP_MEMBER b_scan_local(M.a.cfill, M.a.cstart, M.bsum, M.gdev, vl); }
+NDSP_MK(km_scan_bsum) { NDSP_MEMBER b_scan_bsum(M.bsum, M.gdev, nullptr, vl); }
+NDSP_MK(km_scan_add) { NDSP_MEMBER b_scan_add(M.a.cstart, M.bsum, M.gdev, vl); }
+NDSP_MK(km_scatter) { NDSP_MEMBER b_scatter(M.a, vl); }
+NDSP_MK(km_cellsort) { NDSP_MEMBER b_cellsort(M.a, M.gdev, vl); }
+NDSP_MK(km_fill_nbx) { NDSP_MEMBER b_fill_i(M.a.nbx, M.a.n, 0, vl); }
+NDSP_MK(km_nblist_sort) { NDSP_MEMBER b_nblist_sort(M.a, vl); }
+NDSP_MK(km_init_density) { NDSP_MEMBER b_init_density_scalars(M.a.sc, M.bconstmax, M.a.n, vl); }
+NDSP_MK(km_pass_ctl) { NDSP_MEMBER b_pass_ctl(M.a.sc, M.a.n, M.reflect, M.p.maxits, vl); }
+NDSP_MK(km_hmax) { NDSP_MEMBER b_hmax(M.a, M.a.sc, vl); }
+NDSP_MK(km_refresh_hinv) { NDSP_MEMBER b_refresh_hinv(M.a, M.p, vl); }
+NDSP_MK(km_init_force) { NDSP_MEMBER b_init_force_scalars(M.a.sc, vl); }
+NDSP_MK(km_finalize) { NDSP_MEMBER b_finalize(M.a, M.p, vl); }
+NDSP_MK(km_dt) { NDSP_MEMBER b_dt(M.a.sc, M.p, vl); }
+NDSP_MK(km_step_end) { NDSP_MEMBER b_step_end(M.a.sc, vl); }
+// every member's scalars into the batch's contiguous array (one copy to the host)
+NDSP_MK(km_gather_scalars) {
+  NDSP_MEMBER
+  const int* src = (const int*)M.a.sc;
+  int* dst = (int*)M.out;
+  for (int w = threadIdx.x; w < (int)(sizeof(DScal) / sizeof(int)); w += blockDim.x) dst[w] = src[w];
+}
+// reductions: which[0] = h-rho pass (skipped once finished), 1 = force, 2 = finalize
+__global__ void km_red_final(const MArgs* __restrict__ ma, const int* __restrict__ off, int nm, RedSpec spec,
+                             int which) {
+  NDSP_MEMBER
+  const int nb = which == 0 ? M.nb_nr : which == 1 ? M.nb_force : M.nb_256;
+  b_red_final(M.a.red, M.a.rcap, nb, spec, M.a.sc, which == 0 ? (const int*)&M.a.sc->pc_done : nullptr, vl);
+}
+template <typename PT>
+NDSP_MK(km_pack_density) { NDSP_MEMBER b_pack_density<PT>(M.a, M.p, vl); }
+template <typename PT>
+NDSP_MK(km_pack_force) { NDSP_MEMBER b_pack_force<PT>(M.a, M.p, vl); }
+template <int NDIM>
+__global__ __launch_bounds__(64) void km_nblist(const MArgs* __restrict__ ma, const int* __restrict__ off, int nm) {
+  NDSP_MEMBER b_nblist<NDIM>(M.a, M.p, M.gdev, vl);
+}
+template <int NDIM, typename PT, bool DUST, int TB>
+__global__ __launch_bounds__(TB) void km_nr_pass(const MArgs* __restrict__ ma, const int* __restrict__ off, int nm) {
+  NDSP_MEMBER b_nr_pass<NDIM, PT, DUST, TB>(M.a, M.p, M.gdev, M.list_a, M.list_b, vl);
+}
+template <int NDIM, typename PT, int PHYS, int TB>
+__global__ __launch_bounds__(TB) void km_force(const MArgs* __restrict__ ma, const int* __restrict__ off, int nm) {
+  NDSP_MEMBER b_force<NDIM, PT, PHYS, TB>(M.a, M.p, M.gdev, vl);
+}
+
 static int make_tables(int ikernel, int ndim, std::vector<double>& tab, double& radkern, double& dq2) {
   const int n = NDSP_IKERN + 1;
   tab.assign(3 * n, 0.0);
@@ -2584,6 +2774,12 @@ int ndsp_download(ndsp_ctx* c, const char* name, double* out, int32_t* width) {
     if (width) *width = nd;
     return 0;
   }
+  if (!strcmp(name, "slot")) {
+    // where each particle sits in device memory (the cell order of the last reorder)
+    for (int s = 0; s < n; s++) out[uid[s]] = s;
+    if (width) *width = 1;
+    return 0;
+  }
   if (!strcmp(name, "numneigh") || !strcmp(name, "nits")) {
     std::vector<int> ib(n);
     HIPCHK(hipMemcpy(ib.data(), !strcmp(name, "nits") ? c->a.nits : c->a.numneigh, sizeof(int) * n,
@@ -2773,40 +2969,215 @@ int ndsp_step(ndsp_ctx* c, ndsp_stats* stats, ndsp_dt* dtout) {
 // per half step covering every member, members on their own streams
 // ===========================================================================
 
+// A group of batch members that share dimension, physics, neighbour-list use
+// and walls: one launch per phase covers all of them.
+struct BGroup {
+  int ndim = 0, phys = 0, dust = 0, lists = 0, reflect = 0;
+  std::vector<int> mem;
+  MArgs* args = nullptr;
+  // workgroup offsets per launch shape (device; nm + 1 entries each)
+  enum { S_256, S_64, S_ONE, S_BIN, S_FILL, S_SCAN, S_CAP, S_NR, S_FORCE, S_HMAX, NSHAPE };
+  int* off[NSHAPE] = {};
+  int tot[NSHAPE] = {};
+};
+
 struct ndsp_batch {
   std::vector<ndsp_ctx*> sims;
   hipStream_t st = nullptr;          // stream the graphs are launched on
   hipGraphExec_t ga = nullptr, gb = nullptr;
   bool valid = false;
   DScal* hs = nullptr;               // pinned host copies of every member's scalars
-  std::vector<hipEvent_t> ev;        // fork/join events of the recording
+  DScal* dsc = nullptr;              // the members' scalars gathered on the device
   std::vector<int> caps;             // members' key-space sizes at recording
+  std::vector<BGroup> groups;
+  int prec = 0;
+  int tb = 256;                      // workgroup size of the pair kernels in batch launches
 };
 
-// records part(c) of every member into one graph: each member's launches go
-// to its own stream (forked from and joined back to the batch stream) and
-// end with a copy of its scalars into the pinned host array
-static int batch_capture(ndsp_batch* b, hipGraphExec_t& exec, int (*part)(ndsp_ctx*)) {
+static void batch_free_groups(ndsp_batch* b) {
+  for (auto& g : b->groups) {
+    if (g.args) hipFree(g.args);
+    for (int k = 0; k < BGroup::NSHAPE; k++)
+      if (g.off[k]) hipFree(g.off[k]);
+  }
+  b->groups.clear();
+}
+
+// groups, per-member launch arguments and workgroup offsets of every shape
+static int batch_build(ndsp_batch* b) {
+  batch_free_groups(b);
+  const int ns = (int)b->sims.size();
+  for (int s = 0; s < ns; s++) {
+    ndsp_ctx* c = b->sims[s];
+    const int phys = c->p.imhd ? 1 : c->p.idust ? 2 : 0;
+    BGroup* g = nullptr;
+    for (auto& q : b->groups)
+      if (q.ndim == c->p.ndim && q.phys == phys && q.lists == c->p.use_list && q.reflect == (int)c->reflect)
+        g = &q;
+    if (!g) {
+      b->groups.emplace_back();
+      g = &b->groups.back();
+      g->ndim = c->p.ndim;
+      g->phys = phys;
+      g->dust = c->p.idust != 0;
+      g->lists = c->p.use_list;
+      g->reflect = c->reflect;
+    }
+    g->mem.push_back(s);
+  }
+  for (auto& g : b->groups) {
+    const int nm = (int)g.mem.size();
+    std::vector<MArgs> ma(nm);
+    std::vector<int> off[BGroup::NSHAPE];
+    for (auto& o : off) o.assign(1, 0);
+    for (int q = 0; q < nm; q++) {
+      ndsp_ctx* c = b->sims[g.mem[q]];
+      const int n = c->n;
+      const long long cap = (1LL << c->cap_bits) + 1;
+      const int capb = (int)std::min<long long>(nblk(cap), 1 << 20);
+      MArgs& M = ma[q];
+      memset(&M, 0, sizeof(M));
+      M.a = c->a;
+      M.p = c->p;
+      M.gdev = c->gdev;
+      M.box = c->g;
+      M.part = c->part;
+      M.bsum = c->bsum;
+      M.list_a = c->list_a;
+      M.list_b = c->list_b;
+      M.out = b->dsc + g.mem[q];
+      M.bconstmax = c->bconstmax;
+      M.cap_bits = c->cap_bits;
+      M.reflect = c->reflect;
+      M.nbp = (int)std::min<unsigned>(c->nb_part, nblk(n));
+      M.nb_nr = (int)nblk(n, b->tb);
+      M.nb_force = (int)nblk(n, b->tb);
+      M.nb_256 = (int)nblk(n);
+      const int cnt[BGroup::NSHAPE] = {(int)nblk(n), (int)nblk(n, 64), 1, M.nbp, std::min(capb, 4096),
+                                       (int)((cap + BS * 4 - 1) / (BS * 4)), capb, M.nb_nr, M.nb_force,
+                                       (int)std::min(1024u, nblk(n))};
+      for (int k = 0; k < BGroup::NSHAPE; k++) off[k].push_back(off[k].back() + cnt[k]);
+    }
+    HIPCHK(hipMalloc((void**)&g.args, sizeof(MArgs) * nm));
+    HIPCHK(hipMemcpy(g.args, ma.data(), sizeof(MArgs) * nm, hipMemcpyHostToDevice));
+    for (int k = 0; k < BGroup::NSHAPE; k++) {
+      HIPCHK(hipMalloc((void**)&g.off[k], sizeof(int) * (nm + 1)));
+      HIPCHK(hipMemcpy(g.off[k], off[k].data(), sizeof(int) * (nm + 1), hipMemcpyHostToDevice));
+      g.tot[k] = off[k].back();
+    }
+  }
+  return 0;
+}
+
+// launch of a batch kernel over shape k of group g
+#define BLAUNCH(kern, g, k, threads, ...)                                                                   \
+  hipLaunchKernelGGL(kern, dim3((g).tot[k]), dim3(threads), 0, b->st, (const MArgs*)(g).args,               \
+                     (const int*)(g).off[k], (int)(g).mem.size(), ##__VA_ARGS__)
+
+// first half of a step for every member (as step_part_a), then the scalars
+static int batch_part_a(ndsp_batch* b) {
+  const bool fp64 = b->prec == NDSP_PREC_FP64;
+  for (auto& g : b->groups) {
+    BLAUNCH(km_predict, g, BGroup::S_256, BS);
+    BLAUNCH(km_err_reset, g, BGroup::S_ONE, 64);
+    BLAUNCH(km_boundary_step, g, BGroup::S_256, BS);
+    BLAUNCH(km_binstats, g, BGroup::S_BIN, BS);
+    BLAUNCH(km_binstats_final, g, BGroup::S_ONE, BS);
+    BLAUNCH(km_grid_setup, g, BGroup::S_ONE, 64);
+    BLAUNCH(km_fill_cells, g, BGroup::S_FILL, BS);
+    BLAUNCH(km_cellid, g, BGroup::S_256, BS);
+    BLAUNCH(km_scan_local, g, BGroup::S_SCAN, BS);
+    BLAUNCH(km_scan_bsum, g, BGroup::S_ONE, BS);
+    BLAUNCH(km_scan_add, g, BGroup::S_CAP, BS);
+    BLAUNCH(km_fill_cells, g, BGroup::S_FILL, BS);
+    BLAUNCH(km_scatter, g, BGroup::S_256, BS);
+    BLAUNCH(km_cellsort, g, BGroup::S_CAP, BS);
+    if (fp64) BLAUNCH(km_pack_density<double>, g, BGroup::S_256, BS);
+    else BLAUNCH(km_pack_density<float>, g, BGroup::S_256, BS);
+    if (g.lists) {
+      BLAUNCH(km_fill_nbx, g, BGroup::S_256, BS);
+      dispatch_ndim(g.ndim, [&](auto nd) {
+        constexpr int ND = decltype(nd)::value;
+        BLAUNCH(km_nblist<ND>, g, BGroup::S_64, 64);
+        return 0;
+      });
+      BLAUNCH(km_nblist_sort, g, BGroup::S_256, BS);
+    }
+    BLAUNCH(km_init_density, g, BGroup::S_ONE, 64);
+    const RedSpec spec = red_spec({RED_Q(RED_MAX, RED_F64, stressmax), RED_Q(RED_MAX, RED_F64, hmax_nr),
+                                   RED_Q(RED_MAX, RED_I32, itsmax), RED_Q(RED_MIN, RED_I32, nmin),
+                                   RED_Q(RED_MAX, RED_I32, nmax), RED_Q(RED_SUM, RED_U64, pairs_density),
+                                   RED_Q(RED_SUM, RED_U64, ncalc_total)});
+    const int K = g.reflect ? 4 : 3;
+    for (int q = 0; q < K; q++) {
+      dispatch_ndim(g.ndim, [&](auto nd) {
+        constexpr int ND = decltype(nd)::value;
+        if (b->tb == 64) {
+          if (fp64) { if (g.dust) BLAUNCH((km_nr_pass<ND, double, true, 64>), g, BGroup::S_NR, 64); else BLAUNCH((km_nr_pass<ND, double, false, 64>), g, BGroup::S_NR, 64); }
+          else { if (g.dust) BLAUNCH((km_nr_pass<ND, float, true, 64>), g, BGroup::S_NR, 64); else BLAUNCH((km_nr_pass<ND, float, false, 64>), g, BGroup::S_NR, 64); }
+        } else {
+          if (fp64) { if (g.dust) BLAUNCH((km_nr_pass<ND, double, true, 256>), g, BGroup::S_NR, 256); else BLAUNCH((km_nr_pass<ND, double, false, 256>), g, BGroup::S_NR, 256); }
+          else { if (g.dust) BLAUNCH((km_nr_pass<ND, float, true, 256>), g, BGroup::S_NR, 256); else BLAUNCH((km_nr_pass<ND, float, false, 256>), g, BGroup::S_NR, 256); }
+        }
+        return 0;
+      });
+      BLAUNCH(km_red_final, g, BGroup::S_ONE, BS, spec, 0);
+      BLAUNCH(km_pass_ctl, g, BGroup::S_ONE, 64);
+      BLAUNCH(km_hmax, g, BGroup::S_HMAX, BS);
+      BLAUNCH(km_refresh_hinv, g, BGroup::S_256, BS);
+    }
+    BLAUNCH(km_gather_scalars, g, BGroup::S_ONE, 64);
+  }
+  HIPCHK(hipMemcpyAsync(b->hs, b->dsc, sizeof(DScal) * b->sims.size(), hipMemcpyDeviceToHost, b->st));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// second half of a step for every member (as step_part_b), then the scalars
+static int batch_part_b(ndsp_batch* b) {
+  const bool fp64 = b->prec == NDSP_PREC_FP64;
+  for (auto& g : b->groups) {
+    if (fp64) BLAUNCH(km_pack_force<double>, g, BGroup::S_256, BS);
+    else BLAUNCH(km_pack_force<float>, g, BGroup::S_256, BS);
+    BLAUNCH(km_init_force, g, BGroup::S_ONE, 64);
+    dispatch_ndim(g.ndim, [&](auto nd) {
+      constexpr int ND = decltype(nd)::value;
+#define NDSP_FORCE_B(PTT, TBB)                                                                        \
+  do {                                                                                                \
+    if (g.phys == 1) BLAUNCH((km_force<ND, PTT, 1, TBB>), g, BGroup::S_FORCE, TBB);                   \
+    else if (g.phys == 2) BLAUNCH((km_force<ND, PTT, 2, TBB>), g, BGroup::S_FORCE, TBB);              \
+    else BLAUNCH((km_force<ND, PTT, 0, TBB>), g, BGroup::S_FORCE, TBB);                               \
+  } while (0)
+      if (b->tb == 64) { if (fp64) NDSP_FORCE_B(double, 64); else NDSP_FORCE_B(float, 64); }
+      else { if (fp64) NDSP_FORCE_B(double, 256); else NDSP_FORCE_B(float, 256); }
+#undef NDSP_FORCE_B
+      return 0;
+    });
+    BLAUNCH(km_red_final, g, BGroup::S_ONE, BS,
+            red_spec({RED_Q(RED_MAX, RED_F64, vsigmax), RED_Q(RED_MIN, RED_F64, dtcourant),
+                      RED_Q(RED_SUM, RED_U64, pairs_force)}), 1);
+    BLAUNCH(km_finalize, g, BGroup::S_256, BS);
+    BLAUNCH(km_red_final, g, BGroup::S_ONE, BS,
+            red_spec({RED_Q(RED_MAX, RED_F64, fhmax), RED_Q(RED_MIN, RED_F64, dtdrag)}), 2);
+    BLAUNCH(km_dt, g, BGroup::S_ONE, 64);
+    BLAUNCH(km_correct, g, BGroup::S_256, BS);
+    BLAUNCH(km_boundary_step, g, BGroup::S_256, BS);
+    BLAUNCH(km_step_end, g, BGroup::S_ONE, 64);
+    BLAUNCH(km_gather_scalars, g, BGroup::S_ONE, 64);
+  }
+  HIPCHK(hipMemcpyAsync(b->hs, b->dsc, sizeof(DScal) * b->sims.size(), hipMemcpyDeviceToHost, b->st));
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+static int batch_capture(ndsp_batch* b, hipGraphExec_t& exec, int (*part)(ndsp_batch*)) {
   if (exec) {
     hipGraphExecDestroy(exec);
     exec = nullptr;
   }
-  const int ns = (int)b->sims.size();
   hipGraph_t graph = nullptr;
   HIPCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
-  int err = 0;
-  HIPCHK(hipEventRecord(b->ev[0], b->st));
-  for (int s = 0; s < ns && !err; s++) {
-    ndsp_ctx* c = b->sims[s];
-    if (hipStreamWaitEvent(c->st, b->ev[0], 0) != hipSuccess) { err = set_err(NDSP_ERR_HIP, "fork"); break; }
-    c->capturing = true;
-    err = part(c);
-    c->capturing = false;
-    if (err) break;
-    if (hipMemcpyAsync(&b->hs[s], c->a.sc, sizeof(DScal), hipMemcpyDeviceToHost, c->st) != hipSuccess ||
-        hipEventRecord(b->ev[1 + s], c->st) != hipSuccess || hipStreamWaitEvent(b->st, b->ev[1 + s], 0) != hipSuccess)
-      err = set_err(NDSP_ERR_HIP, "join");
-  }
+  const int err = part(b);
   const hipError_t ce = hipStreamEndCapture(b->st, &graph);
   if (err) {
     if (graph) hipGraphDestroy(graph);
@@ -2822,17 +3193,19 @@ static int batch_capture(ndsp_batch* b, hipGraphExec_t& exec, int (*part)(ndsp_c
 int ndsp_batch_create(ndsp_ctx* const* ctxs, int32_t nsim, ndsp_batch** out) {
   if (!ctxs || !out || nsim < 1) return set_err(NDSP_ERR_BAD_ARG, "batch_create: bad arguments");
   *out = nullptr;
-  for (int s = 0; s < nsim; s++)
+  for (int s = 0; s < nsim; s++) {
     if (!ctxs[s]) return set_err(NDSP_ERR_BAD_ARG, "batch_create: null member");
+    if (ctxs[s]->prec != ctxs[0]->prec) return set_err(NDSP_ERR_BAD_ARG, "batch_create: members differ in precision");
+  }
   ndsp_batch* b = new ndsp_batch();
   b->sims.assign(ctxs, ctxs + nsim);
+  b->prec = ctxs[0]->prec;
   bool ok = hipStreamCreate(&b->st) == hipSuccess;
   ok &= hipHostMalloc((void**)&b->hs, sizeof(DScal) * nsim, hipHostMallocDefault) == hipSuccess;
-  b->ev.resize(nsim + 1, nullptr);
-  for (auto& e : b->ev) ok &= hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+  ok &= hipMalloc((void**)&b->dsc, sizeof(DScal) * nsim) == hipSuccess;
   if (!ok) {
     ndsp_batch_destroy(b);
-    return set_err(NDSP_ERR_ALLOC, "batch_create: stream, events or pinned memory");
+    return set_err(NDSP_ERR_ALLOC, "batch_create: stream or scalar buffers");
   }
   *out = b;
   return 0;
@@ -2843,18 +3216,18 @@ int ndsp_batch_destroy(ndsp_batch* b) {
   if (b->st) hipStreamSynchronize(b->st);
   if (b->ga) hipGraphExecDestroy(b->ga);
   if (b->gb) hipGraphExecDestroy(b->gb);
-  for (auto e : b->ev)
-    if (e) hipEventDestroy(e);
+  batch_free_groups(b);
   if (b->hs) hipHostFree(b->hs);
+  if (b->dsc) hipFree(b->dsc);
   if (b->st) hipStreamDestroy(b->st);
   delete b;
   return 0;
 }
 
 // One leapfrog step of every member (each as ndsp_step would take it): the
-// two halves of all members are replayed as two graphs, members running
-// concurrently on their own streams; members whose h-rho iteration needs
-// more passes than recorded complete it on the host path in between.
+// two halves of all members are replayed as two graphs in which each phase
+// is one launch over all members of a group; members whose h-rho iteration
+// needs more passes than recorded complete it on the host path in between.
 int ndsp_batch_step(ndsp_batch* b, ndsp_stats* stats, ndsp_dt* dt) {
   if (!b) return set_err(NDSP_ERR_BAD_ARG, "null batch");
   const int ns = (int)b->sims.size();
@@ -2870,17 +3243,22 @@ int ndsp_batch_step(ndsp_batch* b, ndsp_stats* stats, ndsp_dt* dt) {
   }
   for (int s = 0; s < ns; s++) HIPCHK(hipStreamSynchronize(b->sims[s]->st));
   if (!b->valid) {
-    if (int e = batch_capture(b, b->ga, step_part_a)) return e;
-    if (int e = batch_capture(b, b->gb, step_part_b)) return e;
+    if (int e = batch_build(b)) return e;
+    if (int e = batch_capture(b, b->ga, batch_part_a)) return e;
+    if (int e = batch_capture(b, b->gb, batch_part_b)) return e;
     b->caps.resize(ns);
     for (int s = 0; s < ns; s++) b->caps[s] = b->sims[s]->cap_bits;
     b->valid = true;
   }
   HIPCHK(hipGraphLaunch(b->ga, b->st));
   HIPCHK(hipStreamSynchronize(b->st));
-  for (int s = 0; s < ns; s++)
+  for (int s = 0; s < ns; s++) {
+    // every member has been binned (as do_linklist leaves it): the next
+    // periodic reorder is due by the member's own step count
+    b->sims[s]->grid_valid = true;
     if (!b->hs[s].pc_done)
       if (int e = density_finish(b->sims[s])) return e;
+  }
   HIPCHK(hipGraphLaunch(b->gb, b->st));
   HIPCHK(hipStreamSynchronize(b->st));
   int first_err = 0;

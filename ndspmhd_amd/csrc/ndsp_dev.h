@@ -213,8 +213,13 @@ __device__ __forceinline__ T pown_t(T x) {
 // contiguous range of the (spatially sorted) particles: neighbour gathers of
 // adjacent tiles then hit the same L2.  A bijection of [0, gridDim.x).
 #define NDSP_NXCD 8
-__device__ __forceinline__ int xcd_tile() {
-  const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+// virtual launch position: block vl.b of vl.nb (blockIdx/gridDim for a
+// launch of one simulation; a block of one member in a launch over a batch)
+struct VL {
+  int b, nb;
+};
+__device__ __forceinline__ int xcd_tile(VL vl) {
+  const int nb = vl.nb, b = vl.b;
   const int x = b % NDSP_NXCD, q = b / NDSP_NXCD;
   int start = 0;
   for (int y = 0; y < x; y++) start += (nb - y + NDSP_NXCD - 1) / NDSP_NXCD;

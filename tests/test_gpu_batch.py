@@ -8,7 +8,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 FIELDS = ("x", "vel", "Bevol", "en", "hh", "alpha", "psi", "rho", "force",
-          "dBevoldt", "dudt", "numneigh", "nits")
+          "dBevoldt", "dudt", "numneigh", "nits", "slot")
 
 
 def _sweep(nsim, seed=11):

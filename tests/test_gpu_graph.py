@@ -12,7 +12,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 FIELDS = ("x", "vel", "Bevol", "en", "hh", "alpha", "psi", "dustevol", "rho",
-          "force", "dBevoldt", "dudt", "numneigh", "nits")
+          "force", "dBevoldt", "dudt", "numneigh", "nits", "slot")
 
 
 def _run_pair(o, st, prec, nsteps):
