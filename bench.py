@@ -163,11 +163,8 @@ def run_sweep(nsim, steps):
         batch = nd.Batch(sims) if mode != "one_by_one" else None
         if mode == "batched_tuned":
             t0 = time.perf_counter()
-            choices = batch.autotune(reps=2)
+            out["tile_shapes"] = batch.autotune(reps=3)
             out["autotune_s"] = time.perf_counter() - t0
-            out["tile_shapes"] = {"force_workgroup_64": sum(c["force_workgroup"] == 64 for c in choices),
-                                  "density_workgroup_64": sum(c["density_workgroup"] == 64 for c in choices),
-                                  "cells_per_radius_1": sum(c["cells_per_radius"] == 1 for c in choices)}
         for _ in range(2):  # warmup (records the graphs)
             if batch:
                 batch.step()

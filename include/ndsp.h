@@ -175,13 +175,19 @@ int ndsp_set_graph(ndsp_ctx *ctx, int32_t on);
  * runs them one after another from multi/ sweep descriptions).  A batch
  * references existing contexts (it does not own them); ndsp_batch_step takes
  * one leapfrog step of every member, exactly as ndsp_step would, replaying
- * both halves of all members as two recorded HIP graphs in which the members
- * run concurrently.  stats and dt receive one entry per member (may be
+ * both halves of all members as two recorded HIP graphs in which each phase
+ * is one launch over all members.  stats and dt receive one entry per member (may be
  * NULL).  Members may not time kernels or use the h predictor. */
 typedef struct ndsp_batch ndsp_batch;
 int ndsp_batch_create(ndsp_ctx *const *ctxs, int32_t nsim, ndsp_batch **out);
 int ndsp_batch_destroy(ndsp_batch *batch);
 int ndsp_batch_step(ndsp_batch *batch, ndsp_stats *stats, ndsp_dt *dt);
+/* Tile shapes of a batch (no counterpart in the reference): the workgroup
+ * size of the merged pair kernels (64 or 256) and the cells per search radius
+ * of every member (1 or 2), each timed over reps batch steps; every member's
+ * state is restored after each trial.  choice[0] = workgroup size,
+ * choice[1] = cells per radius. */
+int ndsp_batch_autotune(ndsp_batch *batch, int32_t reps, int32_t *choice);
 
 /* ---- h-predictor MLP (no counterpart in the reference; its seed for the
  * h-rho iteration is h = h_in + dt dh/dt, src/stepND_leapfrog_mhd.f90:153).

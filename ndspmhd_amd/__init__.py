@@ -37,7 +37,7 @@ SYMBOLS = ("ndsp_create", "ndsp_destroy", "ndsp_upload", "ndsp_download",
            "ndsp_neighbours", "ndsp_set_timing", "ndsp_kernel_times", "ndsp_set_graph",
            "ndsp_mlp_config", "ndsp_mlp_set_params", "ndsp_mlp_get_params",
            "ndsp_mlp_forward", "ndsp_mlp_grad", "ndsp_mlp_train",
-           "ndsp_batch_create", "ndsp_batch_destroy", "ndsp_batch_step", "ndsp_autotune",
+           "ndsp_batch_create", "ndsp_batch_destroy", "ndsp_batch_step", "ndsp_batch_autotune", "ndsp_autotune",
            "ndsp_copy_probe",
            "ndsp_last_error")
 MLP_NPARAM = 1313
@@ -90,6 +90,7 @@ def load_library(path: str | None = None):
     L.ndsp_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.POINTER(vp)]
     L.ndsp_batch_destroy.argtypes = [vp]
     L.ndsp_batch_step.argtypes = [vp, ctypes.POINTER(_Stats), ctypes.POINTER(_Dt)]
+    L.ndsp_batch_autotune.argtypes = [vp, ctypes.c_int32, ip]
     L.ndsp_kernel_times.argtypes = [vp, dp, ctypes.c_int32, ip, ctypes.c_char_p, ctypes.c_int32]
     fp = ctypes.POINTER(ctypes.c_float)
     L.ndsp_mlp_config.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
@@ -275,7 +276,7 @@ class Simulation:
 class Batch:
     """Independent simulations stepped together (ndsp_batch_*): both halves of
     every member's leapfrog step replayed as two recorded HIP graphs in which
-    the members run concurrently.  The batch keeps references to the
+    each phase is one launch over all members.  The batch keeps references to the
     Simulation objects; each member's state, stats and dt_info are updated
     as by Simulation.step."""
 
@@ -298,9 +299,12 @@ class Batch:
         for k, sim in enumerate(self.sims):
             sim._stats(self._st[k], self._dt[k])
 
-    def autotune(self, reps: int = 2):
-        """tile shapes of every member (Simulation.autotune)"""
-        return [s.autotune(reps) for s in self.sims]
+    def autotune(self, reps: int = 2) -> dict:
+        """tile shapes of the batch's merged launches, timed over batch steps
+        (ndsp_batch_autotune); every member's state is left unchanged"""
+        ch = np.zeros(2, dtype=np.int32)
+        self._check(load_library().ndsp_batch_autotune(self.h, int(reps), _ip(ch)))
+        return {"workgroup": int(ch[0]), "cells_per_radius": int(ch[1])}
 
     def close(self):
         if getattr(self, "h", None):

@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cfloat>
 #include <climits>
 #include <cmath>
@@ -37,6 +38,7 @@
 #include <string>
 #include <utility>
 #include <vector>
+#include <chrono>
 
 #include "ndsp.h"
 #include "ndsp_kernels.h"
@@ -2020,6 +2022,8 @@ struct MArgs {
   int reflect;
   int nbp;        // binning-statistics blocks
   int nb_nr, nb_force, nb_256;  // blocks of the h-rho, force and elementwise launches
+  double* scratch_d;  // one field of n doubles / ints for reorders
+  int* scratch_i;
 };
 
 __device__ __forceinline__ int member_of(const int* __restrict__ off, int nm, int b) {
@@ -2068,6 +2072,24 @@ NDSP_MK(km_gather_scalars) {
   const int* src = (const int*)M.a.sc;
   int* dst = (int*)M.out;
   for (int w = threadIdx.x; w < (int)(sizeof(DScal) / sizeof(int)); w += blockDim.x) dst[w] = src[w];
+}
+// one field of the periodic reorder (slot k <- sorted position k) of the
+// members flagged in due: phase 0 gathers it into the member's scratch,
+// phase 1 copies it back; fields past F_NPERSIST are the ids and counters
+__global__ void km_reorder(const MArgs* __restrict__ ma, const int* __restrict__ off, int nm,
+                           const int* __restrict__ due, int fi, int phase) {
+  NDSP_MEMBER
+  if (!due[m_]) return;
+  const int k = vl.b * blockDim.x + threadIdx.x;
+  if (k >= M.a.n) return;
+  if (fi < F_NPERSIST) {
+    if (phase == 0) M.scratch_d[k] = M.a.f[fi][M.a.sidx[k]];
+    else M.a.f[fi][k] = M.scratch_d[k];
+  } else {
+    int* v = fi == F_NPERSIST ? M.a.uid : fi == F_NPERSIST + 1 ? M.a.numneigh : M.a.nits;
+    if (phase == 0) M.scratch_i[k] = v[M.a.sidx[k]];
+    else v[k] = M.scratch_i[k];
+  }
 }
 // reductions: which[0] = h-rho pass (skipped once finished), 1 = force, 2 = finalize
 __global__ void km_red_final(const MArgs* __restrict__ ma, const int* __restrict__ off, int nm, RedSpec spec,
@@ -2969,6 +2991,8 @@ int ndsp_step(ndsp_ctx* c, ndsp_stats* stats, ndsp_dt* dtout) {
 // per half step covering every member, members on their own streams
 // ===========================================================================
 
+static int ensure_key_space(ndsp_ctx* c);
+
 // A group of batch members that share dimension, physics, neighbour-list use
 // and walls: one launch per phase covers all of them.
 struct BGroup {
@@ -2979,6 +3003,7 @@ struct BGroup {
   enum { S_256, S_64, S_ONE, S_BIN, S_FILL, S_SCAN, S_CAP, S_NR, S_FORCE, S_HMAX, NSHAPE };
   int* off[NSHAPE] = {};
   int tot[NSHAPE] = {};
+  int* due = nullptr;  // members whose periodic reorder is due (device, nm entries)
 };
 
 struct ndsp_batch {
@@ -2997,6 +3022,7 @@ struct ndsp_batch {
 static void batch_free_groups(ndsp_batch* b) {
   for (auto& g : b->groups) {
     if (g.args) hipFree(g.args);
+    if (g.due) hipFree(g.due);
     for (int k = 0; k < BGroup::NSHAPE; k++)
       if (g.off[k]) hipFree(g.off[k]);
   }
@@ -3053,11 +3079,14 @@ static int batch_build(ndsp_batch* b) {
       M.nb_nr = (int)nblk(n, b->tb);
       M.nb_force = (int)nblk(n, b->tb);
       M.nb_256 = (int)nblk(n);
+      M.scratch_d = c->scratch_d;
+      M.scratch_i = c->scratch_i;
       const int cnt[BGroup::NSHAPE] = {(int)nblk(n), (int)nblk(n, 64), 1, M.nbp, std::min(capb, 4096),
                                        (int)((cap + BS * 4 - 1) / (BS * 4)), capb, M.nb_nr, M.nb_force,
                                        (int)std::min(1024u, nblk(n))};
       for (int k = 0; k < BGroup::NSHAPE; k++) off[k].push_back(off[k].back() + cnt[k]);
     }
+    HIPCHK(hipMalloc((void**)&g.due, sizeof(int) * nm));
     HIPCHK(hipMalloc((void**)&g.args, sizeof(MArgs) * nm));
     HIPCHK(hipMemcpy(g.args, ma.data(), sizeof(MArgs) * nm, hipMemcpyHostToDevice));
     for (int k = 0; k < BGroup::NSHAPE; k++) {
@@ -3170,6 +3199,28 @@ static int batch_part_b(ndsp_batch* b) {
   return 0;
 }
 
+// the periodic reorder of the flagged members (as reorder() does for one
+// context), each field one launch over all members of a group
+static int batch_reorder(ndsp_batch* b, const std::vector<int>& due) {
+  for (auto& g : b->groups) {
+    const int nm = (int)g.mem.size();
+    std::vector<int> d(nm);
+    bool any = false;
+    for (int q = 0; q < nm; q++) any |= (d[q] = due[g.mem[q]]) != 0;
+    if (!any) continue;
+    HIPCHK(hipMemcpyAsync(g.due, d.data(), sizeof(int) * nm, hipMemcpyHostToDevice, b->st));
+    for (int fi = 0; fi < F_NPERSIST + 3; fi++)
+      for (int phase = 0; phase < 2; phase++)
+        BLAUNCH(km_reorder, g, BGroup::S_256, BS, (const int*)g.due, fi, phase);
+    // the host vector d must outlive the copy
+    HIPCHK(hipStreamSynchronize(b->st));
+  }
+  HIPCHK(hipGetLastError());
+  for (size_t s = 0; s < due.size(); s++)
+    if (due[s]) b->sims[s]->grid_valid = false;
+  return 0;
+}
+
 static int batch_capture(ndsp_batch* b, hipGraphExec_t& exec, int (*part)(ndsp_batch*)) {
   if (exec) {
     hipGraphExecDestroy(exec);
@@ -3231,14 +3282,15 @@ int ndsp_batch_destroy(ndsp_batch* b) {
 int ndsp_batch_step(ndsp_batch* b, ndsp_stats* stats, ndsp_dt* dt) {
   if (!b) return set_err(NDSP_ERR_BAD_ARG, "null batch");
   const int ns = (int)b->sims.size();
+  std::vector<int> due(ns, 0);
   for (int s = 0; s < ns; s++) {
     ndsp_ctx* c = b->sims[s];
     if (c->mlp.enabled || c->timing)
       return set_err(NDSP_ERR_BAD_OPTION, "batch_step: members may not time kernels or use the h predictor");
     if (!c->have_rates)
       if (int e = ndsp_derivs(c, nullptr, nullptr)) return e;
-    if (c->grid_valid && c->reorder_every > 0 && c->nsteps % c->reorder_every == 0)
-      if (int e = reorder(c)) return e;
+    if (int e = ensure_key_space(c)) return e;
+    due[s] = c->grid_valid && c->reorder_every > 0 && c->nsteps % c->reorder_every == 0;
     if (b->caps.size() != (size_t)ns || b->caps[s] != c->cap_bits) b->valid = false;
   }
   for (int s = 0; s < ns; s++) HIPCHK(hipStreamSynchronize(b->sims[s]->st));
@@ -3250,6 +3302,8 @@ int ndsp_batch_step(ndsp_batch* b, ndsp_stats* stats, ndsp_dt* dt) {
     for (int s = 0; s < ns; s++) b->caps[s] = b->sims[s]->cap_bits;
     b->valid = true;
   }
+  if (std::find(due.begin(), due.end(), 1) != due.end())
+    if (int e = batch_reorder(b, due)) return e;
   HIPCHK(hipGraphLaunch(b->ga, b->st));
   HIPCHK(hipStreamSynchronize(b->st));
   for (int s = 0; s < ns; s++) {
@@ -3284,24 +3338,79 @@ int ndsp_batch_step(ndsp_batch* b, ndsp_stats* stats, ndsp_dt* dt) {
 // combination is timed over `reps` derivs evaluations (HIP events on the
 // context's stream); the fastest is kept.  The evolved state is saved before
 // and restored after the trials, so tuning leaves results unchanged.
+// A copy of everything a step evolves (device fields in their current
+// order, the particle ids and counters, the device and host scalars), so
+// that trial evaluations can be undone bit for bit.
+struct SavedState {
+  double* buf = nullptr;
+  DScal sc;
+  double time = 0.0, dt = 0.0;
+  int nsteps = 0;
+  bool have_rates = false, grid_valid = false;
+};
+
+static int save_state(ndsp_ctx* c, SavedState& S) {
+  const size_t n = (size_t)c->n;
+  if (hipMalloc(&S.buf, sizeof(double) * n * (F_NPERSIST + 2)) != hipSuccess)
+    return set_err(NDSP_ERR_ALLOC, "state copy");
+  int* si = (int*)(S.buf + n * F_NPERSIST);  // uid, numneigh, nits
+  for (int fi = 0; fi < F_NPERSIST; fi++)
+    HIPCHK(hipMemcpyAsync(S.buf + fi * n, c->a.f[fi], sizeof(double) * n, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(si, c->a.uid, sizeof(int) * n, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(si + n, c->a.numneigh, sizeof(int) * n, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(si + 2 * n, c->a.nits, sizeof(int) * n, hipMemcpyDeviceToDevice, c->st));
+  if (int e = sync_scalars(c, S.sc)) return e;
+  S.time = c->time;
+  S.dt = c->dt;
+  S.nsteps = c->nsteps;
+  S.have_rates = c->have_rates;
+  S.grid_valid = c->grid_valid;
+  return 0;
+}
+
+static int restore_state(ndsp_ctx* c, const SavedState& S) {
+  const size_t n = (size_t)c->n;
+  const int* si = (const int*)(S.buf + n * F_NPERSIST);
+  for (int fi = 0; fi < F_NPERSIST; fi++)
+    HIPCHK(hipMemcpyAsync(c->a.f[fi], S.buf + fi * n, sizeof(double) * n, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->a.uid, si, sizeof(int) * n, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->a.numneigh, si + n, sizeof(int) * n, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->a.nits, si + 2 * n, sizeof(int) * n, hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->a.sc, &S.sc, sizeof(DScal), hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->time = S.time;
+  c->dt = S.dt;
+  c->nsteps = S.nsteps;
+  c->have_rates = S.have_rates;
+  // the restored order may differ from the last binning's
+  c->grid_valid = false;
+  c->graph_valid = false;
+  return 0;
+}
+
+static void free_state(SavedState& S) {
+  if (S.buf) hipFree(S.buf);
+  S.buf = nullptr;
+}
+
+// Key space of a context whose cell size changed (cap_bits reset to 0): the
+// statistics of the current positions, as the first binning would size it.
+static int ensure_key_space(ndsp_ctx* c) {
+  if (c->cap_bits > 0) return 0;
+  hipLaunchKernelGGL(k_binstats, dim3(c->nb_part), dim3(BS), 0, c->st, c->a, c->p.ndim, c->part);
+  hipLaunchKernelGGL(k_binstats_final, dim3(1), dim3(BS), 0, c->st, c->a.sc, c->part, c->nb_part);
+  DScal hs;
+  if (int e = sync_scalars(c, hs)) return e;
+  return size_key_space(c, hs, 1);
+}
+
 int ndsp_autotune(ndsp_ctx* c, int32_t reps, int32_t* choice) {
   if (!c) return set_err(NDSP_ERR_BAD_ARG, "null context");
   if (reps < 1) reps = 1;
-  const int n = c->n;
   const bool timing = c->timing;
   c->timing = false;
-  // save the evolved state and the scalars
-  double* save = nullptr;
-  if (hipMalloc(&save, sizeof(double) * (size_t)n * (F_NPERSIST + 1)) != hipSuccess)
-    return set_err(NDSP_ERR_ALLOC, "autotune: state copy");
-  int* save_i = (int*)(save + (size_t)n * F_NPERSIST);  // numneigh, nits
-  for (int fi = 0; fi < F_NPERSIST; fi++)
-    HIPCHK(hipMemcpyAsync(save + (size_t)fi * n, c->a.f[fi], sizeof(double) * n, hipMemcpyDeviceToDevice, c->st));
-  HIPCHK(hipMemcpyAsync(save_i, c->a.numneigh, sizeof(int) * n, hipMemcpyDeviceToDevice, c->st));
-  HIPCHK(hipMemcpyAsync(save_i + n, c->a.nits, sizeof(int) * n, hipMemcpyDeviceToDevice, c->st));
-  DScal sc0;
-  if (int e = sync_scalars(c, sc0)) { hipFree(save); return e; }
-  const bool had_rates = c->have_rates;
+  SavedState S;
+  if (int e = save_state(c, S)) { free_state(S); c->timing = timing; return e; }
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
@@ -3340,22 +3449,77 @@ int ndsp_autotune(ndsp_ctx* c, int32_t reps, int32_t* choice) {
   c->tb_nr = bn;
   c->p.cell_sub = bc;
   c->cap_bits = 0;  // the key space follows the chosen cell size at the next binning
-  for (int fi = 0; fi < F_NPERSIST; fi++)
-    hipMemcpyAsync(c->a.f[fi], save + (size_t)fi * n, sizeof(double) * n, hipMemcpyDeviceToDevice, c->st);
-  hipMemcpyAsync(c->a.numneigh, save_i, sizeof(int) * n, hipMemcpyDeviceToDevice, c->st);
-  hipMemcpyAsync(c->a.nits, save_i + n, sizeof(int) * n, hipMemcpyDeviceToDevice, c->st);
-  hipMemcpyAsync(c->a.sc, &sc0, sizeof(DScal), hipMemcpyHostToDevice, c->st);
-  hipStreamSynchronize(c->st);
-  hipFree(save);
-  c->have_rates = had_rates;
-  c->grid_valid = false;
-  c->graph_valid = false;
+  const int re = restore_state(c, S);
+  free_state(S);
   c->timing = timing;
   if (err) return err;
+  if (re) return re;
   if (choice) {
     choice[0] = bf;
     choice[1] = bn;
     choice[2] = bc;
+  }
+  return 0;
+}
+
+// Autotuner of a batch's tile shapes: the workgroup size of the merged pair
+// kernels (64 or 256) and the cells per search radius of every member (1 or
+// 2), each timed over `reps` batch steps after a recorded warm-up step.  All
+// members' states are restored after every trial, so tuning leaves the
+// results of later steps as they would be with the chosen shapes.
+int ndsp_batch_autotune(ndsp_batch* b, int32_t reps, int32_t* choice) {
+  if (!b) return set_err(NDSP_ERR_BAD_ARG, "null batch");
+  if (reps < 1) reps = 1;
+  const int ns = (int)b->sims.size();
+  for (int s = 0; s < ns; s++)
+    if (!b->sims[s]->have_rates)
+      if (int e = ndsp_derivs(b->sims[s], nullptr, nullptr)) return e;
+  std::vector<SavedState> S(ns);
+  int err = 0;
+  for (int s = 0; s < ns && !err; s++) err = save_state(b->sims[s], S[s]);
+  const int tbs[2] = {64, 256}, subs[2] = {1, 2};
+  int bt = b->tb, bc = b->sims[0]->p.cell_sub;
+  double best = 1e300;
+  auto restore_all = [&]() {
+    int e = 0;
+    for (int s = 0; s < ns; s++)
+      if (int r = restore_state(b->sims[s], S[s])) e = e ? e : r;
+    return e;
+  };
+  for (int a = 0; a < 2 && !err; a++)
+    for (int q = 0; q < 2 && !err; q++) {
+      b->tb = tbs[a];
+      for (int s = 0; s < ns; s++) {
+        b->sims[s]->p.cell_sub = subs[q];
+        b->sims[s]->cap_bits = 0;
+      }
+      b->valid = false;
+      err = ndsp_batch_step(b, nullptr, nullptr);  // sizes, records and warms up
+      if (!err) err = restore_all();
+      if (err) break;
+      const auto t0 = std::chrono::steady_clock::now();
+      for (int r = 0; r < reps && !err; r++) err = ndsp_batch_step(b, nullptr, nullptr);
+      const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      if (!err) err = restore_all();
+      if (!err && t < best) {
+        best = t;
+        bt = tbs[a];
+        bc = subs[q];
+      }
+    }
+  b->tb = bt;
+  for (int s = 0; s < ns; s++) {
+    b->sims[s]->p.cell_sub = bc;
+    b->sims[s]->cap_bits = 0;  // sized again at the next step
+  }
+  b->valid = false;
+  const int re = restore_all();
+  for (auto& x : S) free_state(x);
+  if (err) return err;
+  if (re) return re;
+  if (choice) {
+    choice[0] = bt;
+    choice[1] = bc;
   }
   return 0;
 }

@@ -91,3 +91,36 @@ def test_autotune_keeps_state_and_parity():
         a, b = sim.get(f), orc.get(f)
         err = np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
         assert err <= (1e-9 if f != "x" else 1e-11), (f, err)
+
+
+def test_batch_autotune_keeps_state_and_parity():
+    """the batch tuner restores every member bit for bit (layout included);
+    the tuned batch still matches the oracle"""
+    import ndspmhd_amd as nd
+    import oracle
+    from ndspmhd_amd.setups import brio_wu_1d, shock_tube_2d
+    cfgs = [brio_wu_1d(300, hfact=1.3, gamma=5.0 / 3.0), shock_tube_2d(500, hfact=1.1)]
+    members = [nd.Simulation(o, st, precision=1) for o, st in cfgs]
+    orcs = [oracle.Oracle(o, st) for o, st in cfgs]
+    for m, orc in zip(members, orcs):
+        m.derivs()
+        orc.derivs()
+    batch = nd.Batch(members)
+    before = [{f: m.get(f).copy() for f in FIELDS} for m in members]
+    choice = batch.autotune(reps=1)
+    print("batch tile shapes", choice, flush=True)
+    assert choice["workgroup"] in (64, 256) and choice["cells_per_radius"] in (1, 2)
+    for m, b in zip(members, before):
+        for f in FIELDS:
+            assert np.array_equal(b[f], m.get(f)), f
+    for step in range(4):
+        for m, orc in zip(members, orcs):
+            m.set_dt(orc.scalar("dt"), orc.scalar("time"))
+            orc.step()
+        batch.step()
+    for m, orc in zip(members, orcs):
+        for f in ("x", "vel", "Bevol", "en", "hh", "rho", "force"):
+            a, b = m.get(f), orc.get(f)
+            err = np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+            assert err <= (1e-9 if f != "x" else 1e-11), (m.opts.ndim, f, err)
+    batch.close()
