@@ -95,8 +95,10 @@ int ndsp_upload(ndsp_ctx *ctx, const double *x, const double *vel,
  * arrays: x vel pmass hh en Bevol alpha psi dustevol rho gradh gradgradh
  * drhodt dhdt dens pr spsound uu Bfield force dudt dendt dBevoldt daldt
  * dpsidt divB curlB gradpsi dustfrac rhogas rhodust ddustevoldt deltav
- * numneigh nits, and slot (the particle's current position in device
- * memory, which periodic reorders change).  *width receives 1, 3 or ndim. */
+ * del2u graddivv (graddivv of hydro and dust runs only, the runs whose dumps
+ * hold it; zero for MHD) numneigh nits, and slot (the particle's current
+ * position in device memory, which periodic reorders change).  *width
+ * receives 1, 3 or ndim. */
 int ndsp_download(ndsp_ctx *ctx, const char *name, double *out, int32_t *width);
 
 /* Replaces set_linklist (src/linkND.f90:45) together with the ghost set-up
@@ -133,6 +135,20 @@ int ndsp_set_dt(ndsp_ctx *ctx, double dt, double time);
  * out[17] = ekin etherm emag etot momx momy momz momtot angx angy angz angtot
  * rhomin rhomax rhomean totmassgas totmassdust. */
 int ndsp_conserved(ndsp_ctx *ctx, double *out);
+
+/* The numbers of one .ev line, as evwrite computes them
+ * (src/evwrite_mhd.f90:27; loop over the real particles :124-245, columns
+ * :247-308), reduced on the device in FP64.  cols (25 doubles) receives, in
+ * the order evwrite writes them: t ekin etherm emag epot etot momtot angtot
+ * rhomax rhomean dt, then for MHD emagp crosshel betamhdmin betamhdav divBav
+ * divBmax divBtot fdotBav fdotBmax force_err_av force_err_max omegamhdav
+ * omegamhdmax fracdivBok (25 columns), for one-fluid dust ekiny totmassgas
+ * totmassdust (14), otherwise ekiny dmomtot (13); *ncols receives the count.
+ * t and dt are the caller's, as evwrite takes its argument t and the module
+ * dt.  epot is 0 (no gravity or external forces on this path); the fdotB and
+ * force_err columns are 0 because the tensor-form Lorentz force
+ * (imagforce 2) leaves fmag zero (src/ratesND_mhd.f90:202,2621-2626). */
+int ndsp_ev_columns(ndsp_ctx *ctx, double t, double dt, double *cols, int32_t *ncols);
 
 /* Neighbour sets at the current h (test support; the reference's
  * equivalent is the brute-force check_neighbourlist,

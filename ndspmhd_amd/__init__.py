@@ -26,14 +26,14 @@ PREC_MIXED = 0
 PREC_FP64 = 1
 
 FIELDS3 = ("vel", "Bevol", "alpha", "Bfield", "force", "dBevoldt", "daldt", "curlB",
-           "gradpsi", "deltav")
+           "gradpsi", "deltav", "graddivv")
 CONSERVED = ("ekin", "etherm", "emag", "etot", "momx", "momy", "momz", "momtot",
              "angx", "angy", "angz", "angtot", "rhomin", "rhomax", "rhomean",
              "totmassgas", "totmassdust")
 # symbols include/ndsp.h declares
 SYMBOLS = ("ndsp_create", "ndsp_destroy", "ndsp_upload", "ndsp_download",
            "ndsp_set_linklist", "ndsp_iterate_density", "ndsp_get_rates",
-           "ndsp_derivs", "ndsp_step", "ndsp_set_dt", "ndsp_conserved",
+           "ndsp_derivs", "ndsp_step", "ndsp_set_dt", "ndsp_conserved", "ndsp_ev_columns",
            "ndsp_neighbours", "ndsp_set_timing", "ndsp_kernel_times", "ndsp_set_graph",
            "ndsp_mlp_config", "ndsp_mlp_set_params", "ndsp_mlp_get_params",
            "ndsp_mlp_forward", "ndsp_mlp_grad", "ndsp_mlp_train",
@@ -82,6 +82,7 @@ def load_library(path: str | None = None):
     L.ndsp_step.argtypes = [vp, ctypes.POINTER(_Stats), ctypes.POINTER(_Dt)]
     L.ndsp_set_dt.argtypes = [vp, ctypes.c_double, ctypes.c_double]
     L.ndsp_conserved.argtypes = [vp, dp]
+    L.ndsp_ev_columns.argtypes = [vp, ctypes.c_double, ctypes.c_double, dp, ip]
     L.ndsp_neighbours.argtypes = [vp, ctypes.c_int32, ip, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
     L.ndsp_set_timing.argtypes = [vp, ctypes.c_int32]
     L.ndsp_set_graph.argtypes = [vp, ctypes.c_int32]
@@ -148,6 +149,7 @@ class Simulation:
         a = {k: np.ascontiguousarray(st[k], dtype=np.float64)
              for k in ("x", "vel", "pmass", "hh", "en", "Bevol", "alpha", "psi", "dustevol")}
         it = np.ascontiguousarray(st["itype"], dtype=np.int32)
+        self.itype = it.copy()
         self._check(load_library().ndsp_upload(self.h, *[_dp(a[k]) for k in (
             "x", "vel", "pmass", "hh", "en", "Bevol", "alpha", "psi", "dustevol")], _ip(it)))
 
@@ -199,6 +201,15 @@ class Simulation:
         out = np.empty(len(CONSERVED))
         self._check(load_library().ndsp_conserved(self.h, _dp(out)))
         return dict(zip(CONSERVED, out))
+
+    def ev_columns(self, t: float, dt: float) -> np.ndarray:
+        """the numbers of one .ev line at time t with timestep dt, in the
+        order evwrite writes them (ndsp_ev_columns; ndspmhd_amd.evfile names
+        and formats them)"""
+        out = np.zeros(25)
+        nc = ctypes.c_int32()
+        self._check(load_library().ndsp_ev_columns(self.h, float(t), float(dt), _dp(out), ctypes.byref(nc)))
+        return out[: nc.value].copy()
 
     def neighbours(self, sym: bool = False) -> np.ndarray:
         L = load_library()

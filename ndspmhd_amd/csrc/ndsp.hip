@@ -1359,10 +1359,10 @@ __device__ __forceinline__ void b_force(KArgs a, Phys p, const Grid* __restrict_
     const double rad = p.radkern * a.sc->hmax_nr;
     if (!(rad < p.boxmin)) err = NDSP_ERR_CELLS;
 
-    Acc<PT> fx, fy, fz, dudt, dbx, dby, dbz, divb, cbx, cby, cbz, gpx, gpy, gpz, del2u, deps;
+    Acc<PT> fx, fy, fz, dudt, dbx, dby, dbz, divb, cbx, cby, cbz, gpx, gpy, gpz, del2u, deps, gdx, gdy, gdz;
     fx.zero(); fy.zero(); fz.zero(); dudt.zero(); dbx.zero(); dby.zero(); dbz.zero();
     divb.zero(); cbx.zero(); cby.zero(); cbz.zero(); gpx.zero(); gpy.zero(); gpz.zero();
-    del2u.zero(); deps.zero();
+    del2u.zero(); deps.zero(); gdx.zero(); gdy.zero(); gdz.zero();
 
     const bool uselist = p.use_list && a.nbc[k] >= 0 && !a.sc->list_invalid;
     struct FRec {
@@ -1549,6 +1549,13 @@ __device__ __forceinline__ void b_force(KArgs a, Phys p, const Grid* __restrict_
       fz.add(f[2]);
       dudt.add(du);
       if (p.iavlim[1] > 0) del2u.add(mj * rho1j * ((uui - j2.x) * rinv) * grkerni);
+      if (PHYS != 1) {
+        // the "curl v" accumulation of the dissipation-switch block
+        // (src/ratesND_mhd.f90:1653), written to dumps of non-MHD runs
+        gdx.add(mj * (dvel[0] - dvdotr) * grkerni);
+        gdy.add(mj * (dvel[1] - dvdotr) * grkerni);
+        gdz.add(mj * (dvel[2] - dvdotr) * grkerni);
+      }
     };
     if (!err) {
       if (uselist) for_each_listed_pf<NDIM>(a, g, k, a.nbc[k] + a.nbx[k], loadj, body);
@@ -1570,6 +1577,11 @@ __device__ __forceinline__ void b_force(KArgs a, Phys p, const Grid* __restrict_
     a.f[F_GPZ][i] = gpz.value();
     a.f[F_DEL2U][i] = del2u.value();
     a.f[F_DEPS][i] = deps.value();
+    if (PHYS != 1) {
+      a.f[F_GDVX][i] = gdx.value();
+      a.f[F_GDVY][i] = gdy.value();
+      a.f[F_GDVZ][i] = gdz.value();
+    }
   }
   if (err) atomicCAS(&a.sc->err, 0, err);
   {
@@ -1885,6 +1897,149 @@ __global__ void k_conserved_final(const double* part, int nb, double* out, int n
   v[11] = sqrt(v[8] * v[8] + v[9] * v[9] + v[10] * v[10]);
   v[14] = v[14] / (double)n;
   for (int q = 0; q < NCONS; q++) out[q] = v[q];
+}
+
+// ===========================================================================
+// the quantities of one .ev line, src/evwrite_mhd.f90:124-248, FP64
+// ===========================================================================
+
+// slots of the per-block partials: sums, then minima, then maxima
+enum {
+  EV_EKIN, EV_ETHERM, EV_EMAG, EV_EMAGP, EV_MOMX, EV_MOMY, EV_MOMZ, EV_DMOMX, EV_DMOMY, EV_DMOMZ,
+  EV_ANGX, EV_ANGY, EV_ANGZ, EV_EKINY, EV_MGAS, EV_MDUST, EV_RHOSUM, EV_BETASUM, EV_DIVBSUM,
+  EV_DIVBTOT, EV_OMEGASUM, EV_NDIVBOK, EV_CROSSHEL, EV_NSUM,
+  EV_RHOMIN = EV_NSUM, EV_BETAMIN, EV_NMIN_END,
+  EV_RHOMAX = EV_NMIN_END, EV_DIVBMAX, EV_OMEGAMAX, EV_NQ
+};
+
+__device__ __forceinline__ double ev_combine(int q, double x, double y) {
+  return q < EV_NSUM ? x + y : q < EV_NMIN_END ? fmin(x, y) : fmax(x, y);
+}
+__device__ __forceinline__ double ev_identity(int q) {
+  return q < EV_NSUM ? 0.0 : q < EV_NMIN_END ? DBL_MAX : -DBL_MAX;
+}
+
+// per-particle terms of evwrite's loop over the real particles
+// (src/evwrite_mhd.f90:124-245) folded into per-block partials; the
+// magnetic-force diagnostics (fdotB, force_err) are not summed: with the
+// tensor-form Lorentz force fmag stays zero (src/ratesND_mhd.f90:202,
+// 2621-2626), so both are zero for every particle
+__global__ __launch_bounds__(BS) void k_ev_sums(KArgs a, Phys p, double* __restrict__ part) {
+  __shared__ double sm[BS / 64][EV_NQ];
+  double v[EV_NQ];
+#pragma unroll
+  for (int q = 0; q < EV_NQ; q++) v[q] = ev_identity(q);
+  double** f = a.f;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
+    const double m = f[F_M][i], rho = f[F_RHO][i];
+    const double vx = f[F_VX][i], vy = f[F_VY][i], vz = f[F_VZ][i];
+    v[EV_MOMX] += m * vx; v[EV_MOMY] += m * vy; v[EV_MOMZ] += m * vz;
+    v[EV_DMOMX] += m * f[F_FX][i]; v[EV_DMOMY] += m * f[F_FY][i]; v[EV_DMOMZ] += m * f[F_FZ][i];
+    const double x = f[F_X][i], y = p.ndim >= 2 ? f[F_Y][i] : 0.0, z = p.ndim >= 3 ? f[F_Z][i] : 0.0;
+    if (p.ndim == 3) {
+      v[EV_ANGX] += m * (y * vz - z * vy);
+      v[EV_ANGY] += m * (z * vx - x * vz);
+      v[EV_ANGZ] += m * (x * vy - y * vx);
+    } else if (p.ndim == 2) {
+      v[EV_ANGZ] += m * (x * vy - y * vx);
+    }
+    v[EV_EKIN] += 0.5 * m * (vx * vx + vy * vy + vz * vz);
+    if (p.idust) {
+      const double eps = f[F_DUSTFRAC][i], dterm = 1.0 - eps;
+      const double dx = f[F_DVX][i], dy = f[F_DVY][i], dz = f[F_DVZ][i];
+      const double ekdv = 0.5 * m * eps * dterm * (dx * dx + dy * dy + dz * dz);
+      v[EV_EKIN] += ekdv;
+      v[EV_EKINY] += ekdv;
+      v[EV_ETHERM] += m * f[F_UU][i] * dterm;
+      v[EV_MGAS] += m * dterm;
+      v[EV_MDUST] += m * eps;
+    } else {
+      if (p.ndim >= 2) v[EV_EKINY] += 0.5 * m * vx * vx;
+      v[EV_ETHERM] += m * f[F_UU][i];
+    }
+    if (p.imhd) {
+      const double bx = f[F_BX][i], by = f[F_BY][i], bz = f[F_BZ][i];
+      const double b2 = bx * bx + by * by + bz * bz, bmag = sqrt(b2);
+      const double divb = fabs(f[F_DIVB][i]);
+      v[EV_EMAG] += 0.5 * m * b2 / rho;
+      v[EV_EMAGP] += 0.5 * m * (bx * bx + by * by) / rho;
+      const double beta = b2 < DBL_MIN ? 0.0 : f[F_PR][i] / (0.5 * b2);
+      v[EV_BETASUM] += beta;
+      v[EV_BETAMIN] = fmin(v[EV_BETAMIN], beta);
+      v[EV_DIVBMAX] = fmax(v[EV_DIVBMAX], divb);
+      v[EV_DIVBSUM] += divb;
+      v[EV_DIVBTOT] += m * divb / rho;
+      const double om = bmag < 1.0e-8 ? 0.0 : divb * f[F_H][i] / bmag;
+      if (om < 1.0e-2) v[EV_NDIVBOK] += 1.0;
+      v[EV_OMEGAMAX] = fmax(v[EV_OMEGAMAX], om);
+      v[EV_OMEGASUM] += om;
+      v[EV_CROSSHEL] += m * (vx * (bx / rho) + vy * (by / rho) + vz * (bz / rho));
+    }
+    v[EV_RHOSUM] += rho;
+    v[EV_RHOMIN] = fmin(v[EV_RHOMIN], rho);
+    v[EV_RHOMAX] = fmax(v[EV_RHOMAX], rho);
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < EV_NQ; q++) {
+    double x = v[q];
+    for (int o = 32; o > 0; o >>= 1) x = ev_combine(q, x, __shfl_xor(x, o, 64));
+    if (lane == 0) sm[w][q] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < EV_NQ) {
+    const int q = threadIdx.x;
+    double x = sm[0][q];
+    for (int k = 1; k < BS / 64; k++) x = ev_combine(q, x, sm[k][q]);
+    part[(size_t)blockIdx.x * EV_NQ + q] = x;
+  }
+}
+
+// folds the block partials (in block order) and forms the columns of the
+// .ev line in the order evwrite writes them (src/evwrite_mhd.f90:247-308)
+__global__ void k_ev_final(const double* __restrict__ part, int nb, int n, Phys p, double t, double dt,
+                           double* __restrict__ out) {
+  if (threadIdx.x | blockIdx.x) return;
+  double v[EV_NQ];
+  for (int q = 0; q < EV_NQ; q++) {
+    double x = ev_identity(q);
+    for (int b = 0; b < nb; b++) x = ev_combine(q, x, part[(size_t)b * EV_NQ + q]);
+    v[q] = x;
+  }
+  const double epot = 0.0;  // no gravity or external potentials on this path
+  const double ekin = v[EV_EKIN], etherm = v[EV_ETHERM], emag = v[EV_EMAG];
+  // iprterm 0: the thermal energy counts (src/evwrite_mhd.f90:247-248)
+  const double etot = ekin + emag + epot + etherm;
+  const double momtot = sqrt(v[EV_MOMX] * v[EV_MOMX] + v[EV_MOMY] * v[EV_MOMY] + v[EV_MOMZ] * v[EV_MOMZ]);
+  const double dmomtot = sqrt(v[EV_DMOMX] * v[EV_DMOMX] + v[EV_DMOMY] * v[EV_DMOMY] + v[EV_DMOMZ] * v[EV_DMOMZ]);
+  const double angtot = sqrt(v[EV_ANGX] * v[EV_ANGX] + v[EV_ANGY] * v[EV_ANGY] + v[EV_ANGZ] * v[EV_ANGZ]);
+  const double fn = (double)n;
+  int k = 0;
+  const double common[11] = {t, ekin, etherm, emag, epot, etot, momtot, angtot, v[EV_RHOMAX], v[EV_RHOSUM] / fn, dt};
+  for (int q = 0; q < 11; q++) out[k++] = common[q];
+  if (p.imhd) {
+    out[k++] = v[EV_EMAGP];
+    out[k++] = v[EV_CROSSHEL];
+    out[k++] = v[EV_BETAMIN];
+    out[k++] = v[EV_BETASUM] / fn;
+    out[k++] = v[EV_DIVBSUM] / fn;
+    out[k++] = v[EV_DIVBMAX];
+    out[k++] = v[EV_DIVBTOT];
+    out[k++] = 0.0;  // fdotBav
+    out[k++] = 0.0;  // fdotBmax
+    out[k++] = 0.0;  // force_err_av
+    out[k++] = 0.0;  // force_err_max
+    out[k++] = v[EV_OMEGASUM] / fn;
+    out[k++] = v[EV_OMEGAMAX];
+    out[k++] = 100.0 * v[EV_NDIVBOK] / fn;
+  } else if (p.idust) {
+    out[k++] = v[EV_EKINY];
+    out[k++] = v[EV_MGAS];
+    out[k++] = v[EV_MDUST];
+  } else {
+    out[k++] = v[EV_EKINY];
+    out[k++] = dmomtot;
+  }
 }
 
 // ===========================================================================
@@ -2778,7 +2933,7 @@ static const struct { const char* name; int f0; int w; } kFieldMap[] = {
     {"dendt", F_DENDT, 1}, {"dBevoldt", F_DBX, 3}, {"daldt", F_DA0, 3}, {"dpsidt", F_DPSIDT, 1},
     {"divB", F_DIVB, 1}, {"curlB", F_CBX, 3}, {"gradpsi", F_GPX, 3}, {"dustfrac", F_DUSTFRAC, 1},
     {"rhogas", F_RHOGAS, 1}, {"rhodust", F_RHODUST, 1}, {"ddustevoldt", F_DEPS, 1},
-    {"deltav", F_DVX, 3}, {"del2u", F_DEL2U, 1}};
+    {"deltav", F_DVX, 3}, {"del2u", F_DEL2U, 1}, {"graddivv", F_GDVX, 3}};
 
 int ndsp_download(ndsp_ctx* c, const char* name, double* out, int32_t* width) {
   if (!c || !name || !out) return set_err(NDSP_ERR_BAD_ARG, "download: null argument");
@@ -3560,6 +3715,21 @@ int ndsp_conserved(ndsp_ctx* c, double* out) {
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, c->cons_out, sizeof(double) * NCONS, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int ndsp_ev_columns(ndsp_ctx* c, double t, double dt, double* cols, int32_t* ncols) {
+  if (!c || !cols) return set_err(NDSP_ERR_BAD_ARG, "null argument");
+  if (!c->have_rates) return set_err(NDSP_ERR_BAD_ARG, "ev_columns: call derivs first");
+  const int nb = std::min(c->nb_part, (int)nblk(c->n));
+  const int nc = c->p.imhd ? 25 : c->p.idust ? 14 : 13;
+  hipLaunchKernelGGL(k_ev_sums, dim3(nb), dim3(BS), 0, c->st, c->a, c->p, c->part);
+  hipLaunchKernelGGL(k_ev_final, dim3(1), dim3(1), 0, c->st, (const double*)c->part, nb, c->n, c->p, t, dt,
+                     c->cons_out);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(cols, c->cons_out, sizeof(double) * nc, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (ncols) *ncols = nc;
   return 0;
 }
 

@@ -15,6 +15,7 @@ enum Field {
   F_FX, F_FY, F_FZ, F_DUDT, F_DENDT, F_DBX, F_DBY, F_DBZ, F_DA0, F_DA1, F_DA2,
   F_DPSIDT, F_DIVB, F_CBX, F_CBY, F_CBZ, F_GPX, F_GPY, F_GPZ, F_DEPS, F_DVX,
   F_DVY, F_DVZ, F_DEL2U,
+  F_GDVX, F_GDVY, F_GDVZ,  // graddivv of the rates (hydro and dust; output only)
   F_MLPCORR,  // last step's log(h_converged / h_predictor), an MLP feature
   F_NPERSIST,  // fields above are carried by a reorder
   // step in-copies (src/stepND_leapfrog_mhd.f90:70-100)

@@ -99,6 +99,14 @@ module ndsp_capi
        real(c_double), intent(out) :: out(17)
     end function ndsp_conserved
 
+    integer(c_int) function ndsp_ev_columns(ctx, t, dt, cols, ncols) bind(c, name='ndsp_ev_columns')
+       import :: c_int, c_ptr, c_double, c_int32_t
+       type(c_ptr), value :: ctx
+       real(c_double), value :: t, dt
+       real(c_double), intent(out) :: cols(25)
+       integer(c_int32_t), intent(out) :: ncols
+    end function ndsp_ev_columns
+
     type(c_ptr) function ndsp_last_error() bind(c, name='ndsp_last_error')
        import :: c_ptr
     end function ndsp_last_error

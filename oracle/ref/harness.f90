@@ -17,7 +17,7 @@
 ! with a non-zero status.
 !
 ! Input / output: little-endian stream files, layout documented in
-! tests/golden/refio.py.
+! tests/golden/refio.py; the .ev text of the state goes to <output>.ev.
 !------------------------------------------------------------------------------
 program ndsp_ref_harness
  use dimen_mhd
@@ -47,7 +47,7 @@ program ndsp_ref_harness
  integer :: npg,nps
  integer, allocatable :: listneigh(:),pg(:,:),ps(:,:)
  real, allocatable :: dustin(:),zeros(:),zeros3(:,:)
- real :: rij2,q2i,q2j,hi1,hi21,hj1
+ real :: rij2,q2i,q2j,hi1,hi21,hj1,etot,momtot
  integer(8) :: count0,count1,count_rate
  real, dimension(ndim) :: dx
  character(len=256) :: infile,outfile
@@ -214,6 +214,7 @@ program ndsp_ref_harness
  else
     write(22) zeros,zeros,zeros,zeros,zeros3
  endif
+ write(22) graddivv(1:3,1:np)
  write(22) numneigh(1:np)
 
  if (mode == 2) then
@@ -264,6 +265,13 @@ program ndsp_ref_harness
     write(22) ps(1:3,1:nps)
  endif
  close(22)
+!
+!--the .ev line of this state, written by the reference's own evwrite
+!  (src/evwrite_mhd.f90:27) at time 0 with the input dt
+!
+ open(unit=ievfile,file=trim(outfile)//'.ev',status='replace',form='formatted')
+ call evwrite(time,etot,momtot)
+ close(ievfile)
 
 end program ndsp_ref_harness
 

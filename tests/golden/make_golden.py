@@ -52,6 +52,9 @@ def cases():
     o = o.replace(iavlim=(2, 1, 1), xmin=(-0.5, -0.5, -0.5), xmax=(0.5, 0.5, 0.5))
     st["x"] -= 0.5
     yield "mhd3d_resist", o, st
+    o, st = periodic_mhd_box(8, seed=41, vel_seed=42, jitter=0.2)
+    st = perturb_mhd(st, 43)
+    yield "hydro3d_nr", o.replace(imhd=0), st
     o, st = dusty_box(8, ndim=3, seed=31)
     rng = np.random.default_rng(32)
     st["vel"][:] = 0.05 * rng.standard_normal(st["vel"].shape)

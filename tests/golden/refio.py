@@ -28,7 +28,7 @@ OUT_FIELDS = [("x", None), ("hh", 1), ("rho", 1), ("gradh", 1), ("gradgradh", 1)
               ("uu", 1), ("psi", 1), ("force", 3), ("dudt", 1), ("dendt", 1),
               ("dBevoldt", 3), ("daldt", 3), ("dpsidt", 1), ("divB", 1),
               ("curlB", 3), ("gradpsi", 3), ("dustfrac", 1), ("rhogas", 1),
-              ("rhodust", 1), ("ddustevoldt", 1), ("deltav", 3)]
+              ("rhodust", 1), ("ddustevoldt", 1), ("deltav", 3), ("graddivv", 3)]
 SCALARS = ["dtcourant", "dtforce", "dtdrag", "dtav", "vsig2max", "hhmax"]
 
 
@@ -89,5 +89,7 @@ def run_reference(binary, opts, st, mode=1, timeout=600):
             raise RuntimeError("reference harness failed (%d):\n%s\n%s"
                                % (p.returncode, p.stdout[-4000:], p.stderr[-4000:]))
         out = read_output(fout, opts.ndim, mode)
+        with open(fout + ".ev") as f:
+            out["ev"] = f.read()
         out["log"] = p.stdout
         return out
