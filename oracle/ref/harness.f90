@@ -17,7 +17,8 @@
 ! with a non-zero status.
 !
 ! Input / output: little-endian stream files, layout documented in
-! tests/golden/refio.py; the .ev text of the state goes to <output>.ev.
+! tests/golden/refio.py; the .ev text of the state goes to <output>.ev and
+! the options, as write_infile writes an input file, to <output>.in.
 !------------------------------------------------------------------------------
 program ndsp_ref_harness
  use dimen_mhd
@@ -39,6 +40,7 @@ program ndsp_ref_harness
  use mem_allocation, only:alloc
  use dust,           only:init_drag
  use get_neighbour_lists, only:get_neighbour_list_partial
+ use infiles,        only:write_infile
  implicit none
  integer, parameter :: maxopt = 32
  integer :: iopt(maxopt)
@@ -272,6 +274,26 @@ program ndsp_ref_harness
  open(unit=ievfile,file=trim(outfile)//'.ev',status='replace',form='formatted')
  call evwrite(time,etot,momtot)
  close(ievfile)
+!
+!--the options as the reference's own write_infile writes them
+!  (src/readwrite_infile.f90:35)
+!
+ call write_infile(trim(outfile)//'.in')
+!--and values that take the writer's other branches (es10.3 below 1e-3 and
+!  from 1e4, g16.9 from 10 with all zeros stripped, long and negative values)
+ psep = 36.87
+ tmax = 0.075
+ tout = 1.23456789012
+ polyk = 2.5e-4
+ C_cour = -0.3
+ hsoft = 12345.678
+ etamhd = 1.0e-12
+ avdecayconst = 0.0999999
+ beta = 10.0
+ rhocrit = 99.5
+ Kdrag = 20.
+ alphamin = 0.123456789
+ call write_infile(trim(outfile)//'.in2')
 
 end program ndsp_ref_harness
 

@@ -91,5 +91,9 @@ def run_reference(binary, opts, st, mode=1, timeout=600):
         out = read_output(fout, opts.ndim, mode)
         with open(fout + ".ev") as f:
             out["ev"] = f.read()
+        with open(fout + ".in") as f:
+            out["infile"] = f.read()
+        with open(fout + ".in2") as f:
+            out["infile2"] = f.read()
         out["log"] = p.stdout
         return out
