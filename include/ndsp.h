@@ -187,6 +187,13 @@ int ndsp_copy_probe(ndsp_ctx *ctx, int64_t nbytes);
  * launch-by-launch path.  Not used while timing or the h predictor is on. */
 int ndsp_set_graph(ndsp_ctx *ctx, int32_t on);
 
+/* Interval of the periodic reorder (no counterpart in the reference, which
+ * keeps its particles unsorted): every `every` steps (0 = never; default 10)
+ * the evolved state is permuted into the cell order of the last binning, so
+ * that per-particle reads and writes of the pair kernels stay coalesced.
+ * Results differ from another interval only in summation order. */
+int ndsp_set_reorder(ndsp_ctx *ctx, int32_t every);
+
 /* ---- batches of independent simulations (a parameter sweep; the reference
  * runs them one after another from multi/ sweep descriptions).  A batch
  * references existing contexts (it does not own them); ndsp_batch_step takes

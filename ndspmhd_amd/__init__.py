@@ -38,6 +38,7 @@ SYMBOLS = ("ndsp_create", "ndsp_destroy", "ndsp_upload", "ndsp_download",
            "ndsp_mlp_config", "ndsp_mlp_set_params", "ndsp_mlp_get_params",
            "ndsp_mlp_forward", "ndsp_mlp_grad", "ndsp_mlp_train",
            "ndsp_batch_create", "ndsp_batch_destroy", "ndsp_batch_step", "ndsp_batch_autotune", "ndsp_autotune",
+           "ndsp_set_reorder",
            "ndsp_copy_probe",
            "ndsp_last_error")
 MLP_NPARAM = 1313
@@ -86,6 +87,7 @@ def load_library(path: str | None = None):
     L.ndsp_neighbours.argtypes = [vp, ctypes.c_int32, ip, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
     L.ndsp_set_timing.argtypes = [vp, ctypes.c_int32]
     L.ndsp_set_graph.argtypes = [vp, ctypes.c_int32]
+    L.ndsp_set_reorder.argtypes = [vp, ctypes.c_int32]
     L.ndsp_autotune.argtypes = [vp, ctypes.c_int32, ip]
     L.ndsp_copy_probe.argtypes = [vp, ctypes.c_int64]
     L.ndsp_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.POINTER(vp)]
@@ -269,6 +271,11 @@ class Simulation:
     def copy_probe(self, nbytes: int):
         """plain device copy of nbytes (counter calibration, ndsp_copy_probe)"""
         self._check(load_library().ndsp_copy_probe(self.h, int(nbytes)))
+
+    def set_reorder(self, every: int):
+        """permute the state into cell order every `every` steps (0 = never;
+        ndsp_set_reorder)"""
+        self._check(load_library().ndsp_set_reorder(self.h, int(every)))
 
     def set_graph(self, on: bool):
         """replay each step as two recorded HIP graphs (ndsp_set_graph)"""

@@ -499,44 +499,46 @@ __device__ __forceinline__ void for_each_listed_pf(const KArgs& a, const Grid& g
   if (cnt <= 0) return;
   const uint4* L4 = (const uint4*)(a.nbl + (size_t)k * a.maxnb);
   const int ng = (cnt + 3) >> 2;
-  uint4 gc = L4[0];
-  auto r = load((int)(gc.x >> 7));
-  for (int gi = 0; gi < ng; gi++) {
-    const uint4 gn = gi + 1 < ng ? L4[gi + 1] : gc;
-    const unsigned ev[4] = {gc.x, gc.y, gc.z, gc.w};
-    const int mm = min(4, cnt - 4 * gi);
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      if (u < mm) {
-        const unsigned e = ev[u];
-        const bool more = 4 * gi + u + 1 < cnt;
-        const unsigned en = !more ? e : u < 3 ? ev[u + 1] : gn.x;
-        const auto rn = load((int)(en >> 7));
-        const int kk = (int)(e >> 7);
-        const int code = (int)(e & 63u);
-        double xj[3] = {r.p.x, r.p.y, r.p.z};
-        const int flip[3] = {0, 0, 0};
-        bool ok = true;
-        if (code) {
-          // ghost arithmetic of src/ghostND_mhd.f90:180-200; a periodic image
-          // of a particle exactly on the wall does not exist (:172)
-          for (int d = 0; d < NDIM; d++) {
-            const int kd = (code >> (2 * d)) & 3;
-            const double x0 = xj[d];
-            if (kd == 1) {
-              ok = ok && g.xmax[d] - x0 > 0.0;
-              xj[d] = g.xmin[d] + (x0 - g.xmax[d]);
-            } else if (kd == 2) {
-              ok = ok && x0 - g.xmin[d] > 0.0;
-              xj[d] = g.xmax[d] + (x0 - g.xmin[d]);
-            }
-          }
+  // entries are read four at a time into an eight-entry window (c0..c3 the
+  // current ones, n0..n3 the next group) that shifts by one per entry, so
+  // that no entry is picked by a run-time index (an indexed uint4 would live
+  // in scratch memory); the next record is fetched one entry ahead
+  const uint4 g0 = L4[0];
+  const uint4 g1 = ng > 1 ? L4[1] : make_uint4(0u, 0u, 0u, 0u);
+  unsigned c0 = g0.x, c1 = g0.y, c2 = g0.z, c3 = g0.w, n0 = g1.x, n1 = g1.y, n2 = g1.z, n3 = g1.w;
+  auto r = load((int)(c0 >> 7));
+  for (int m = 0; m < cnt; m++) {
+    const unsigned e = c0;
+    const unsigned en = m + 1 < cnt ? c1 : e;
+    const auto rn = load((int)(en >> 7));
+    const int kk = (int)(e >> 7);
+    const int code = (int)(e & 63u);
+    double xj[3] = {r.p.x, r.p.y, r.p.z};
+    const int flip[3] = {0, 0, 0};
+    bool ok = true;
+    if (code) {
+      // ghost arithmetic of src/ghostND_mhd.f90:180-200; a periodic image
+      // of a particle exactly on the wall does not exist (:172)
+      for (int d = 0; d < NDIM; d++) {
+        const int kd = (code >> (2 * d)) & 3;
+        const double x0 = xj[d];
+        if (kd == 1) {
+          ok = ok && g.xmax[d] - x0 > 0.0;
+          xj[d] = g.xmin[d] + (x0 - g.xmax[d]);
+        } else if (kd == 2) {
+          ok = ok && x0 - g.xmin[d] > 0.0;
+          xj[d] = g.xmax[d] + (x0 - g.xmin[d]);
         }
-        if (ok) body(kk, r, xj, flip, code);
-        r = rn;
       }
     }
-    gc = gn;
+    if (ok) body(kk, r, xj, flip, code);
+    r = rn;
+    c0 = c1; c1 = c2; c2 = c3; c3 = n0; n0 = n1; n1 = n2; n2 = n3;
+    if ((m & 3) == 3) {
+      const int gi2 = (m >> 2) + 2;
+      const uint4 t = gi2 < ng ? L4[gi2] : make_uint4(0u, 0u, 0u, 0u);
+      n0 = t.x; n1 = t.y; n2 = t.z; n3 = t.w;
+    }
   }
 }
 
@@ -2071,6 +2073,32 @@ __global__ void k_gather_i(const int* __restrict__ in, int* __restrict__ out, co
   if (k < n) out[k] = in[sidx[k]];
 }
 
+// up to REORDER_W fields of one kind per launch: gather into the scratch
+// (slot k <- sidx[k]), then copy back, so the field pointers never change
+#define REORDER_W 8
+template <typename T>
+struct FieldSet {
+  T* f[REORDER_W];
+  int nf;
+};
+template <typename T>
+__global__ void k_gather_set(FieldSet<T> fs, T* __restrict__ scratch, const int* __restrict__ sidx, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int i = sidx[k];
+#pragma unroll
+  for (int q = 0; q < REORDER_W; q++)
+    if (q < fs.nf) scratch[(size_t)q * n + k] = fs.f[q][i];
+}
+template <typename T>
+__global__ void k_copy_set(FieldSet<T> fs, const T* __restrict__ scratch, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+#pragma unroll
+  for (int q = 0; q < REORDER_W; q++)
+    if (q < fs.nf) fs.f[q][k] = scratch[(size_t)q * n + k];
+}
+
 // ===========================================================================
 // neighbour-set enumeration (test support)
 // ===========================================================================
@@ -2348,7 +2376,7 @@ struct ndsp_ctx {
   double bconstmax;
   hipStream_t st = nullptr;
   KArgs a;
-  double* scratch_d = nullptr;  // one field of n doubles for reorders
+  double* scratch_d = nullptr;  // REORDER_W fields of n doubles for reorders
   int* scratch_i = nullptr;
   int* bsum = nullptr;
   int* list_a = nullptr;        // active lists of the h-rho passes
@@ -2673,17 +2701,22 @@ static int reorder(ndsp_ctx* c) {
   KArgs& a = c->a;
   const int n = c->n;
   TScope t(c, "reorder");
-  for (int fi = 0; fi < F_NPERSIST; fi++) {
-    hipLaunchKernelGGL(k_gather_d, dim3(nblk(n)), dim3(BS), 0, c->st, (const double*)a.f[fi], c->scratch_d,
+  for (int f0 = 0; f0 < F_NPERSIST; f0 += REORDER_W) {
+    FieldSet<double> fs{};
+    fs.nf = std::min(REORDER_W, F_NPERSIST - f0);
+    for (int q = 0; q < fs.nf; q++) fs.f[q] = a.f[f0 + q];
+    hipLaunchKernelGGL(k_gather_set<double>, dim3(nblk(n)), dim3(BS), 0, c->st, fs, c->scratch_d,
                        (const int*)a.sidx, n);
-    hipLaunchKernelGGL(k_copy_d, dim3(nblk(n)), dim3(BS), 0, c->st, (const double*)c->scratch_d, a.f[fi], n);
+    hipLaunchKernelGGL(k_copy_set<double>, dim3(nblk(n)), dim3(BS), 0, c->st, fs, (const double*)c->scratch_d, n);
   }
-  int* ia[3] = {a.uid, a.numneigh, a.nits};
-  for (int q = 0; q < 3; q++) {
-    hipLaunchKernelGGL(k_gather_i, dim3(nblk(n)), dim3(BS), 0, c->st, (const int*)ia[q], c->scratch_i,
-                       (const int*)a.sidx, n);
-    hipLaunchKernelGGL(k_copy_i, dim3(nblk(n)), dim3(BS), 0, c->st, (const int*)c->scratch_i, ia[q], n);
-  }
+  FieldSet<int> is{};
+  is.nf = 3;
+  is.f[0] = a.uid;
+  is.f[1] = a.numneigh;
+  is.f[2] = a.nits;
+  int* si = (int*)c->scratch_d;
+  hipLaunchKernelGGL(k_gather_set<int>, dim3(nblk(n)), dim3(BS), 0, c->st, is, si, (const int*)a.sidx, n);
+  hipLaunchKernelGGL(k_copy_set<int>, dim3(nblk(n)), dim3(BS), 0, c->st, is, (const int*)si, n);
   HIPCHK(hipGetLastError());
   c->grid_valid = false;
   return 0;
@@ -2791,7 +2824,7 @@ int ndsp_create(const int32_t* iopt, const double* dopt, int64_t npart, int32_t 
     hipMemset(a.f[fi], 0, sizeof(double) * n);
   }
   bool ok = true;
-  ok &= hipMalloc(&c->scratch_d, sizeof(double) * n) == hipSuccess;
+  ok &= hipMalloc(&c->scratch_d, sizeof(double) * n * REORDER_W) == hipSuccess;
   ok &= hipMalloc(&c->scratch_i, sizeof(int) * n) == hipSuccess;
   ok &= hipMalloc(&a.numneigh, sizeof(int) * n) == hipSuccess;
   ok &= hipMalloc(&a.nits, sizeof(int) * n) == hipSuccess;
@@ -3697,6 +3730,12 @@ int ndsp_copy_probe(ndsp_ctx* c, int64_t nbytes) {
   HIPCHK(hipStreamSynchronize(c->st));
   hipFree(src);
   hipFree(dst);
+  return 0;
+}
+
+int ndsp_set_reorder(ndsp_ctx* c, int32_t every) {
+  if (!c || every < 0) return set_err(NDSP_ERR_BAD_ARG, "set_reorder: bad arguments");
+  c->reorder_every = every;
   return 0;
 }
 
