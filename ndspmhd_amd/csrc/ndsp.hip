@@ -580,6 +580,10 @@ __device__ __forceinline__ double min_image_r2(const Grid& g, const double* xa, 
   return r2;
 }
 
+// streaming stores (each output written once per kernel): non-temporal, so
+// that they do not evict the neighbour records the gathers reuse from L2
+__device__ __forceinline__ void st_stream(double* p, double v) { __builtin_nontemporal_store(v, p); }
+
 __device__ __forceinline__ int wave_min_i(int v) {
   for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
   return v;
@@ -1153,11 +1157,11 @@ __device__ __forceinline__ void b_nr_pass(KArgs a, Phys p, const Grid* __restric
     if (done) {
       // ---- results, then conservative->primitive and EOS for this particle ----
       a.f[F_H][i] = h;
-      a.f[F_RHO][i] = rho_v;
-      a.f[F_GRADH][i] = gradh_v;
-      a.f[F_GRADGRADH][i] = ggh_v;
-      a.f[F_DRHODT][i] = drhodt_v;
-      a.f[F_DHDT][i] = dhdt_v;
+      st_stream(&a.f[F_RHO][i], rho_v);
+      st_stream(&a.f[F_GRADH][i], gradh_v);
+      st_stream(&a.f[F_GRADGRADH][i], ggh_v);
+      st_stream(&a.f[F_DRHODT][i], drhodt_v);
+      st_stream(&a.f[F_DHDT][i], dhdt_v);
       a.numneigh[i] = nn;
       a.nits[i] = its;
       double dens;
@@ -1563,26 +1567,26 @@ __device__ __forceinline__ void b_force(KArgs a, Phys p, const Grid* __restrict_
       if (uselist) for_each_listed_pf<NDIM>(a, g, k, a.nbc[k] + a.nbx[k], loadj, body);
       else for_each_candidate_rec<NDIM>(a, g, xi, rad, loadj, body);
     }
-    a.f[F_FX][i] = fx.value();
-    a.f[F_FY][i] = fy.value();
-    a.f[F_FZ][i] = fz.value();
-    a.f[F_DUDT][i] = dudt.value();
-    a.f[F_DBX][i] = dbx.value();
-    a.f[F_DBY][i] = dby.value();
-    a.f[F_DBZ][i] = dbz.value();
-    a.f[F_DIVB][i] = divb.value();
-    a.f[F_CBX][i] = cbx.value();
-    a.f[F_CBY][i] = cby.value();
-    a.f[F_CBZ][i] = cbz.value();
-    a.f[F_GPX][i] = gpx.value();
-    a.f[F_GPY][i] = gpy.value();
-    a.f[F_GPZ][i] = gpz.value();
-    a.f[F_DEL2U][i] = del2u.value();
-    a.f[F_DEPS][i] = deps.value();
+    st_stream(&a.f[F_FX][i], fx.value());
+    st_stream(&a.f[F_FY][i], fy.value());
+    st_stream(&a.f[F_FZ][i], fz.value());
+    st_stream(&a.f[F_DUDT][i], dudt.value());
+    st_stream(&a.f[F_DBX][i], dbx.value());
+    st_stream(&a.f[F_DBY][i], dby.value());
+    st_stream(&a.f[F_DBZ][i], dbz.value());
+    st_stream(&a.f[F_DIVB][i], divb.value());
+    st_stream(&a.f[F_CBX][i], cbx.value());
+    st_stream(&a.f[F_CBY][i], cby.value());
+    st_stream(&a.f[F_CBZ][i], cbz.value());
+    st_stream(&a.f[F_GPX][i], gpx.value());
+    st_stream(&a.f[F_GPY][i], gpy.value());
+    st_stream(&a.f[F_GPZ][i], gpz.value());
+    st_stream(&a.f[F_DEL2U][i], del2u.value());
+    st_stream(&a.f[F_DEPS][i], deps.value());
     if (PHYS != 1) {
-      a.f[F_GDVX][i] = gdx.value();
-      a.f[F_GDVY][i] = gdy.value();
-      a.f[F_GDVZ][i] = gdz.value();
+      st_stream(&a.f[F_GDVX][i], gdx.value());
+      st_stream(&a.f[F_GDVY][i], gdy.value());
+      st_stream(&a.f[F_GDVZ][i], gdz.value());
     }
   }
   if (err) atomicCAS(&a.sc->err, 0, err);
