@@ -1010,10 +1010,14 @@ __device__ __forceinline__ void b_nr_pass(KArgs a, Phys p, const Grid* __restric
   const int persistent = a.sc->pc_persistent;
   const int* __restrict__ list = its0 == 1 ? nullptr : ((its0 & 1) ? list_b : list_a);
   int* __restrict__ next_list = (its0 & 1) ? list_a : list_b;
+  // a workgroup with no particle of this pass skips loading the tables (a
+  // later pass holds only the particles still iterating); it still writes
+  // its neutral reduction partials, which the final fold reads
+  const bool idle = xcd_tile(vl) * (int)blockDim.x >= nlist;
   __shared__ PT tw[NDSP_IKERN + 1];
   __shared__ PT tg[NDSP_IKERN + 1];
   __shared__ PT tgg[NDSP_IKERN + 1];
-  {
+  if (!idle) {
     const PT* src = sizeof(PT) == 4 ? (const PT*)a.tabf : (const PT*)a.tabd;
     for (int t = threadIdx.x; t <= NDSP_IKERN; t += blockDim.x) {
       tw[t] = src[t];
