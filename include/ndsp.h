@@ -82,13 +82,23 @@ int ndsp_create(const int32_t *iopt, const double *dopt, int64_t npart,
 int ndsp_destroy(ndsp_ctx *ctx);
 
 /* Host -> HBM copy of the evolved state (reference layout); itype may be
- * NULL (all gas).  Replaces nothing in the reference (its state lives in
+ * NULL (all gas); itype 1 marks fixed boundary particles (ibound 1 walls,
+ * partners then given by ndsp_set_ireal).  Replaces nothing in the
+ * reference (its state lives in
  * module arrays); the Fortran host calls it after setup and after any host
  * edit of the arrays. */
 int ndsp_upload(ndsp_ctx *ctx, const double *x, const double *vel,
                 const double *pmass, const double *hh, const double *en,
                 const double *Bevol, const double *alpha, const double *psi,
                 const double *dustevol, const int32_t *itype);
+
+/* Partners of the fixed boundary particles (src/set_fixedbound.f90:32-48
+ * sets ireal; used by iterate_density.f90:310-326, conservative2primitive
+ * .f90:425-432 and stepND_leapfrog_mhd.f90:110-186): ireal[i] is the
+ * 0-based index (caller order) of the gas particle that fixed particle i
+ * copies, -1 for particles that are not fixed.  Required after an upload
+ * with itype 1 particles (walls with ibound 1) before the first binning. */
+int ndsp_set_ireal(ndsp_ctx *ctx, const int32_t *ireal);
 
 /* HBM -> host copy of one per-particle field of the real particles, in the
  * caller's particle order and reference layout.  Names as the reference's
@@ -97,8 +107,11 @@ int ndsp_upload(ndsp_ctx *ctx, const double *x, const double *vel,
  * dpsidt divB curlB gradpsi dustfrac rhogas rhodust ddustevoldt deltav
  * del2u graddivv (graddivv of hydro and dust runs only, the runs whose dumps
  * hold it; zero for MHD) numneigh nits, and slot (the particle's current
- * position in device memory, which periodic reorders change).  *width
- * receives 1, 3 or ndim. */
+ * position in device memory, which periodic reorders change).  Bfield and
+ * uu are B and en at the last evaluation (after a step, the predictor's, as
+ * the reference's .ev file and dumps report them; a fixed particle's are
+ * its partner's), Bevol and en the evolved values.  *width receives 1, 3 or
+ * ndim. */
 int ndsp_download(ndsp_ctx *ctx, const char *name, double *out, int32_t *width);
 
 /* Replaces set_linklist (src/linkND.f90:45) together with the ghost set-up

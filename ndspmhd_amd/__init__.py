@@ -38,7 +38,7 @@ SYMBOLS = ("ndsp_create", "ndsp_destroy", "ndsp_upload", "ndsp_download",
            "ndsp_mlp_config", "ndsp_mlp_set_params", "ndsp_mlp_get_params",
            "ndsp_mlp_forward", "ndsp_mlp_grad", "ndsp_mlp_train",
            "ndsp_batch_create", "ndsp_batch_destroy", "ndsp_batch_step", "ndsp_batch_autotune", "ndsp_autotune",
-           "ndsp_set_reorder",
+           "ndsp_set_reorder", "ndsp_set_ireal",
            "ndsp_copy_probe",
            "ndsp_last_error")
 MLP_NPARAM = 1313
@@ -88,6 +88,7 @@ def load_library(path: str | None = None):
     L.ndsp_set_timing.argtypes = [vp, ctypes.c_int32]
     L.ndsp_set_graph.argtypes = [vp, ctypes.c_int32]
     L.ndsp_set_reorder.argtypes = [vp, ctypes.c_int32]
+    L.ndsp_set_ireal.argtypes = [vp, ip]
     L.ndsp_autotune.argtypes = [vp, ctypes.c_int32, ip]
     L.ndsp_copy_probe.argtypes = [vp, ctypes.c_int64]
     L.ndsp_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.POINTER(vp)]
@@ -154,6 +155,9 @@ class Simulation:
         self.itype = it.copy()
         self._check(load_library().ndsp_upload(self.h, *[_dp(a[k]) for k in (
             "x", "vel", "pmass", "hh", "en", "Bevol", "alpha", "psi", "dustevol")], _ip(it)))
+        if "ireal" in st:
+            ir = np.ascontiguousarray(st["ireal"], dtype=np.int32)
+            self._check(load_library().ndsp_set_ireal(self.h, _ip(ir)))
 
     def get(self, name: str) -> np.ndarray:
         w = 3 if name in FIELDS3 else (self.opts.ndim if name == "x" else 1)

@@ -395,14 +395,25 @@ __device__ __forceinline__ void b_pack_force(KArgs a, Phys p, VL vl) {
   int i = a.sidx[k];
   double h = a.f[F_H][i];
   a.P4[k].w = 1.0 / h;
+  // a fixed particle's h was replaced by its partner's after the iteration
+  if (p.fixed_on && p.use_list && a.itype[i] == 1 && h > a.hcap[k]) atomicOr(&a.sc->list_invalid, 1);
   typedef typename V4<PT>::type T4;
   ((T4*)a.G0)[k] = T4{(PT)a.f[F_VX][i], (PT)a.f[F_VY][i], (PT)a.f[F_VZ][i], (PT)a.f[F_M][i]};
   ((T4*)a.G1)[k] = T4{(PT)a.f[F_RHO][i], (PT)a.f[F_PR][i], (PT)a.f[F_CS][i], (PT)a.f[F_GRADH][i]};
   ((T4*)a.G2)[k] = T4{(PT)a.f[F_UU][i], (PT)a.f[F_A0][i], (PT)a.f[F_A1][i], (PT)a.f[F_A2][i]};
-  if (p.imhd)
+  if (p.imhd) {
     ((T4*)a.G3)[k] = T4{(PT)a.f[F_BX][i], (PT)a.f[F_BY][i], (PT)a.f[F_BZ][i], (PT)a.f[F_PSI][i]};
+    // Bfield of conservative2primitive (src/conservative2primitive.f90:152),
+    // kept for the outputs after the corrector has moved B on
+    for (int d = 0; d < 3; d++) a.f[F_BFX + d][i] = a.f[F_BX + d][i];
+  }
   ((T4*)a.G4)[k] = T4{(PT)h, (PT)a.f[F_DENS][i], (PT)a.f[F_DUSTFRAC][i], (PT)a.f[F_RHOGAS][i]};
   if (p.idust) ((PT*)a.G5)[k] = (PT)a.f[F_RHODUST][i];
+  if (p.fixed_on) {
+    const bool fx = a.itype[i] == 1;
+    ((T4*)a.GF)[k] = T4{(PT)a.f[fx ? F_MIMG : F_M][i], (PT)a.f[fx ? F_A0IMG : F_A0][i],
+                        (PT)a.f[fx ? F_A1IMG : F_A1][i], (PT)a.f[fx ? F_A2IMG : F_A2][i]};
+  }
 }
 template <typename PT>
 __global__ void k_pack_force(KArgs a, Phys p) {
@@ -1036,7 +1047,11 @@ __device__ __forceinline__ void b_nr_pass(KArgs a, Phys p, const Grid* __restric
   const PT* De = (const PT*)a.De;
   // this thread's share of the global reductions
   double r_stress = 0.0, r_hmax = 0.0, r_its = 0.0, r_nmin = DBL_MAX, r_nmax = 0.0, r_pairs = 0.0, r_ncalc = 0.0;
-  if (active) {
+  // fixed particles are not iterated (src/iterate_density.f90:166,253) and
+  // take their partner's values after the iteration; the first symmetric pass
+  // still counts their neighbours (src/density_sums.f90:166-169)
+  const bool fixedp = active && p.fixed_on && a.itype[a.sidx[k]] == 1;
+  if (active && (!fixedp || (its0 == 1 && sym))) {
     const int i = a.sidx[k];
     const double4 pk = a.P4[k];
     const double xi[3] = {pk.x, pk.y, pk.z};
@@ -1073,7 +1088,7 @@ __device__ __forceinline__ void b_nr_pass(KArgs a, Phys p, const Grid* __restric
       its++;
       const bool symp = sym && its == its0;
       if (!symp && p.imhd != 0) a.f[F_PSI][i] = 0.0;
-      if (symp && p.imhd == 0) a.f[F_PSI][i] = 0.0;
+      if (symp && p.imhd == 0 && !fixedp) a.f[F_PSI][i] = 0.0;
       const double hi1 = 1.0 / h;
       const double hi21 = mul_exact(hi1, hi1);
       const PT hi1p = (PT)hi1;
@@ -1131,6 +1146,12 @@ __device__ __forceinline__ void b_nr_pass(KArgs a, Phys p, const Grid* __restric
       else for_each_candidate_rec<NDIM>(a, g, xi, rad, loadj, body);
       npairs += (unsigned long long)ngat;
       nn = symp ? nsym : ngat;
+      if (fixedp) {
+        a.numneigh[i] = nn;
+        r_nmin = nn;
+        r_nmax = nn;
+        break;
+      }
       // Newton-Raphson update, src/iterate_density.f90:162-278 (FP64)
       rho_v = rho.value();
       drhodt_v = drhodt.value();
@@ -1328,6 +1349,7 @@ __device__ __forceinline__ void b_force(KArgs a, Phys p, const Grid* __restrict_
   const T4* G3 = (const T4*)a.G3;
   const T4* G4 = (const T4*)a.G4;
   const PT* G5 = (const PT*)a.G5;
+  const T4* GF = (const T4*)a.GF;
   constexpr bool mhd = PHYS == 1;
   constexpr bool dust = PHYS == 2;
   const PT ddq2 = (PT)p.ddq2table, dq2 = (PT)p.dq2table;
@@ -1409,7 +1431,18 @@ __device__ __forceinline__ void b_force(KArgs a, Phys p, const Grid* __restrict_
       const PT rinv = p_rcp(rij);
       PT dr[3] = {0, 0, 0};
       for (int d = 0; d < NDIM; d++) dr[d] = (PT)dx[d] * rinv;
-      const T4 j0 = rj.g0, j1 = rj.g1, j2 = rj.g2, j4 = rj.g4;
+      const T4 j1 = rj.g1, j4 = rj.g4;
+      T4 j0 = rj.g0, j2 = rj.g2;
+      if (p.fixed_on && code != 0) {
+        // the ghosts of src/ghostND_mhd.f90:363-420 are copies made before
+        // conservative2primitive hands a fixed particle its partner's mass
+        // and alpha (src/conservative2primitive.f90:425-432), and keep them
+        const T4 gf = GF[kk];
+        j0.w = gf.x;
+        j2.y = gf.y;
+        j2.z = gf.z;
+        j2.w = gf.w;
+      }
       const PT hj1p = (PT)hj1;
       const PT hfacgrkernj = pown_t<NDIM>(hj1p) * hj1p;
       const PT grkerni = ktab(tg, (PT)q2i, ddq2, dq2) * hfacgrkerni * gradhi;
@@ -1686,6 +1719,14 @@ __device__ __forceinline__ void b_finalize(KArgs a, Phys p, VL vl) {
     } else {
       f[F_DPSIDT][i] = 0.0;
     }
+    if (p.fixed_on && a.itype[i] == 1) {
+      // rates are zero on fixed particles, after their force entered fhmax
+      // (src/ratesND_mhd.f90:949-966)
+      static const int zf[] = {F_FX, F_FY, F_FZ, F_DRHODT, F_DHDT, F_DUDT, F_DENDT, F_DBX, F_DBY, F_DBZ,
+                               F_DA0, F_DA1, F_DA2, F_DPSIDT, F_DIVB, F_CBX, F_CBY, F_CBZ, F_GPX, F_GPY,
+                               F_GPZ};
+      for (int q = 0; q < (int)(sizeof zf / sizeof zf[0]); q++) f[zf[q]][i] = 0.0;
+    }
   }
   double rv[2] = {fh, dtd};
   const int rk[2] = {RED_MAX, RED_MIN};
@@ -1803,6 +1844,65 @@ __global__ void k_predict(KArgs a, Phys p) {
   b_predict(a, p, VL{(int)blockIdx.x, (int)gridDim.x});
 }
 
+// Predictor of the fixed particles, after b_predict has written every
+// slot's step in-copies (src/stepND_leapfrog_mhd.f90:110-143): a fixed
+// particle moves with the starting velocity and force of the particle it
+// copies and holds its own velocity, B, h, energy, switches, psi and dust.
+__device__ __forceinline__ void b_predict_fixed(KArgs a, Phys p, VL vl) {
+#pragma clang fp contract(off)
+  const int i = vl.b * blockDim.x + threadIdx.x;
+  if (!p.fixed_on || i >= a.n || a.itype[i] != 1) return;
+  const double dt = a.sc->dt_cur;
+  double** f = a.f;
+  const int j = a.slot_of[a.ireal[i]];
+  static const int xs[3] = {F_X, F_Y, F_Z}, vs[3] = {F_VX, F_VY, F_VZ}, bs[3] = {F_BX, F_BY, F_BZ};
+  static const int as[3] = {F_A0, F_A1, F_A2};
+  for (int d = 0; d < p.ndim; d++)
+    f[xs[d]][i] = f[F_XIN + d][i] + dt * f[F_VXIN + d][j] + 0.5 * dt * dt * f[F_FXIN + d][j];
+  for (int c = 0; c < 3; c++) {
+    f[vs[c]][i] = f[F_VXIN + c][i];
+    f[bs[c]][i] = f[F_BXIN + c][i];
+    f[as[c]][i] = f[F_A0IN + c][i];
+  }
+  f[F_H][i] = f[F_HIN][i];
+  f[F_EN][i] = f[F_ENIN][i];
+  f[F_PSI][i] = f[F_PSIIN][i];
+  f[F_EPS][i] = f[F_EPSIN][i];
+  f[F_RHOIN][i] = f[F_RHO][i];
+}
+__global__ void k_predict_fixed(KArgs a, Phys p) {
+  b_predict_fixed(a, p, VL{(int)blockIdx.x, (int)gridDim.x});
+}
+
+// After the h-rho iteration a fixed particle becomes a replica of the particle
+// it copies (iterate_density.f90:310-326, then copy_particle from
+// conservative2primitive.f90:425-432): mass, density, h, energies, B,
+// switches, psi, the h terms, pressure and sound speed, and the dust
+// fields; position and velocity stay its own.  Rates are formed afterwards.
+__device__ __forceinline__ void b_fixed_copy(KArgs a, Phys p, VL vl) {
+  const int i = vl.b * blockDim.x + threadIdx.x;
+  if (!p.fixed_on || i >= a.n || a.itype[i] != 1) return;
+  const int j = a.slot_of[a.ireal[i]];
+  a.f[F_MIMG][i] = a.f[F_M][i];
+  a.f[F_A0IMG][i] = a.f[F_A0][i];
+  a.f[F_A1IMG][i] = a.f[F_A1][i];
+  a.f[F_A2IMG][i] = a.f[F_A2][i];
+  static const int cf[] = {F_M, F_RHO, F_H, F_UU, F_EN, F_BX, F_BY, F_BZ, F_A0, F_A1, F_A2, F_PSI,
+                           F_GRADH, F_GRADGRADH, F_CS, F_PR, F_DENS, F_DRHODT, F_DHDT, F_DUSTFRAC,
+                           F_EPS, F_RHOGAS, F_RHODUST, F_DVX, F_DVY, F_DVZ};
+  for (int q = 0; q < (int)(sizeof cf / sizeof cf[0]); q++) a.f[cf[q]][i] = a.f[cf[q]][j];
+}
+__global__ void k_fixed_copy(KArgs a, Phys p) {
+  b_fixed_copy(a, p, VL{(int)blockIdx.x, (int)gridDim.x});
+}
+
+// slot of every caller index (after uploads and reorders)
+__device__ __forceinline__ void b_slot_of(KArgs a, VL vl) {
+  const int s = vl.b * blockDim.x + threadIdx.x;
+  if (s < a.n) a.slot_of[a.uid[s]] = s;
+}
+__global__ void k_slot_of(KArgs a) { b_slot_of(a, VL{(int)blockIdx.x, (int)gridDim.x}); }
+
 __device__ __forceinline__ void b_correct(KArgs a, Phys p, VL vl) {
 #pragma clang fp contract(off)
   int i = vl.b * blockDim.x + threadIdx.x;
@@ -1812,6 +1912,21 @@ __device__ __forceinline__ void b_correct(KArgs a, Phys p, VL vl) {
   const double hdt = 0.5 * dt;
   static const int vs[3] = {F_VX, F_VY, F_VZ}, bs[3] = {F_BX, F_BY, F_BZ}, as[3] = {F_A0, F_A1, F_A2};
   static const int fs[3] = {F_FX, F_FY, F_FZ}, dbs[3] = {F_DBX, F_DBY, F_DBZ}, das[3] = {F_DA0, F_DA1, F_DA2};
+  if (p.fixed_on && a.itype[i] == 1) {
+    // fixed particles get back their own starting values
+    // (src/stepND_leapfrog_mhd.f90:172-186)
+    for (int c = 0; c < 3; c++) {
+      f[vs[c]][i] = f[F_VXIN + c][i];
+      f[bs[c]][i] = f[F_BXIN + c][i];
+      f[as[c]][i] = f[F_A0IN + c][i];
+    }
+    f[F_H][i] = f[F_HIN][i];
+    f[F_EN][i] = f[F_ENIN][i];
+    f[F_PSI][i] = f[F_PSIIN][i];
+    f[F_EPS][i] = f[F_EPSIN][i];
+    f[F_RHO][i] = f[F_RHOIN][i];
+    return;
+  }
   for (int c = 0; c < 3; c++)
     f[vs[c]][i] = (f[F_VXIN + c][i] + hdt * (f[fs[c]][i] + f[F_FXIN + c][i])) / 1.0;
   if (p.imhd != 0)
@@ -1867,7 +1982,7 @@ __global__ void k_conserved(KArgs a, Phys p, double* part) {
       v[1] += m * f[F_UU][i];
     }
     if (p.imhd) {
-      const double bx = f[F_BX][i], by = f[F_BY][i], bz = f[F_BZ][i];
+      const double bx = f[F_BFX][i], by = f[F_BFY][i], bz = f[F_BFZ][i];
       v[2] += 0.5 * m * (bx * bx + by * by + bz * bz) / rho;
     }
     v[12] = fmin(v[12], rho);
@@ -1968,7 +2083,7 @@ __global__ __launch_bounds__(BS) void k_ev_sums(KArgs a, Phys p, double* __restr
       v[EV_ETHERM] += m * f[F_UU][i];
     }
     if (p.imhd) {
-      const double bx = f[F_BX][i], by = f[F_BY][i], bz = f[F_BZ][i];
+      const double bx = f[F_BFX][i], by = f[F_BFY][i], bz = f[F_BFZ][i];
       const double b2 = bx * bx + by * by + bz * bz, bmag = sqrt(b2);
       const double divb = fabs(f[F_DIVB][i]);
       v[EV_EMAG] += 0.5 * m * b2 / rho;
@@ -2234,6 +2349,9 @@ __device__ __forceinline__ int member_of(const int* __restrict__ off, int nm, in
 #define NDSP_MK(name) __global__ void name(const MArgs* __restrict__ ma, const int* __restrict__ off, int nm)
 
 NDSP_MK(km_predict) { NDSP_MEMBER b_predict(M.a, M.p, vl); }
+NDSP_MK(km_predict_fixed) { NDSP_MEMBER b_predict_fixed(M.a, M.p, vl); }
+NDSP_MK(km_fixed_copy) { NDSP_MEMBER b_fixed_copy(M.a, M.p, vl); }
+NDSP_MK(km_slot_of) { NDSP_MEMBER b_slot_of(M.a, vl); }
 NDSP_MK(km_correct) { NDSP_MEMBER b_correct(M.a, M.p, vl); }
 NDSP_MK(km_boundary_step) { NDSP_MEMBER b_boundary(M.a, M.p, M.box, 1, vl); }
 NDSP_MK(km_err_reset) { NDSP_MEMBER if (threadIdx.x == 0) M.a.sc->err = 0; }
@@ -2277,7 +2395,8 @@ __global__ void km_reorder(const MArgs* __restrict__ ma, const int* __restrict__
     if (phase == 0) M.scratch_d[k] = M.a.f[fi][M.a.sidx[k]];
     else M.a.f[fi][k] = M.scratch_d[k];
   } else {
-    int* v = fi == F_NPERSIST ? M.a.uid : fi == F_NPERSIST + 1 ? M.a.numneigh : M.a.nits;
+    int* const iv[5] = {M.a.uid, M.a.numneigh, M.a.nits, M.a.itype, M.a.ireal};
+    int* v = iv[fi - F_NPERSIST];
     if (phase == 0) M.scratch_i[k] = v[M.a.sidx[k]];
     else v[k] = M.scratch_i[k];
   }
@@ -2369,6 +2488,8 @@ static double nexp_of(const Phys& p) {
 
 struct ndsp_ctx {
   int tb_force = 256, tb_nr = 256;  // workgroup sizes of the pair kernels (autotuned)
+  int nfixed = 0;           // fixed boundary particles uploaded
+  bool ireal_set = true;    // their partners have been given
   bool reflect = false;     // some wall reflects
   Grid* gdev = nullptr;     // cell grid of the last binning (device)
   int cap_bits = 0;         // Morton key-space capacity of cstart/cfill
@@ -2517,6 +2638,7 @@ static int size_key_space(ndsp_ctx* c, const DScal& hs, int extra_bits) {
 static int do_linklist(ndsp_ctx* c, int in_step) {
   KArgs& a = c->a;
   const int n = c->n;
+  if (!c->ireal_set) return set_err(NDSP_ERR_BAD_ARG, "fixed particles need their partners: call ndsp_set_ireal");
   hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, c->st, &a.sc->err, 1, 0);
   {
     TScope t(c, "boundary");
@@ -2636,6 +2758,7 @@ static int density_finish(ndsp_ctx* c) {
 static int do_rates(ndsp_ctx* c) {
   KArgs& a = c->a;
   const int n = c->n;
+  if (c->p.fixed_on) hipLaunchKernelGGL(k_fixed_copy, dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p);
   {
     TScope t(c, "pack_force");
     if (c->prec == NDSP_PREC_FP64)
@@ -2718,13 +2841,16 @@ static int reorder(ndsp_ctx* c) {
     hipLaunchKernelGGL(k_copy_set<double>, dim3(nblk(n)), dim3(BS), 0, c->st, fs, (const double*)c->scratch_d, n);
   }
   FieldSet<int> is{};
-  is.nf = 3;
+  is.nf = 5;
   is.f[0] = a.uid;
   is.f[1] = a.numneigh;
   is.f[2] = a.nits;
+  is.f[3] = a.itype;
+  is.f[4] = a.ireal;
   int* si = (int*)c->scratch_d;
   hipLaunchKernelGGL(k_gather_set<int>, dim3(nblk(n)), dim3(BS), 0, c->st, is, si, (const int*)a.sidx, n);
   hipLaunchKernelGGL(k_copy_set<int>, dim3(nblk(n)), dim3(BS), 0, c->st, is, (const int*)si, n);
+  hipLaunchKernelGGL(k_slot_of, dim3(nblk(n)), dim3(BS), 0, c->st, a);
   HIPCHK(hipGetLastError());
   c->grid_valid = false;
   return 0;
@@ -2779,7 +2905,8 @@ int ndsp_create(const int32_t* iopt, const double* dopt, int64_t npart, int32_t 
   else if (precision != NDSP_PREC_MIXED && precision != NDSP_PREC_FP64) bad = "unknown precision";
   int nrefl = 0;
   for (int d = 0; d < p.ndim && bad.empty(); d++) {
-    if (!(ib[d] == 0 || ib[d] == 2 || ib[d] == 3)) bad = "ibound must be 0, 2 or 3";
+    if (!(ib[d] == 0 || ib[d] == 1 || ib[d] == 2 || ib[d] == 3)) bad = "ibound must be 0, 1, 2 or 3";
+    if (ib[d] == 1) p.fixed_on = 1;
     if (ib[d] == 2) nrefl++;
     if (ib[d] >= 2 && !(dopt[16 + d] > dopt[13 + d])) bad = "xmax must exceed xmin";
   }
@@ -2813,7 +2940,8 @@ int ndsp_create(const int32_t* iopt, const double* dopt, int64_t npart, int32_t 
     c->ibound[d] = d < p.ndim ? ib[d] : 0;
     c->xmin[d] = dopt[13 + d];
     c->xmax[d] = dopt[16 + d];
-    c->g.bound[d] = c->ibound[d];
+    // a wall of fixed particles (ibound 1) is open space to the neighbour search
+    c->g.bound[d] = c->ibound[d] == 1 ? 0 : c->ibound[d];
     c->g.xmin[d] = c->xmin[d];
     c->g.xmax[d] = c->xmax[d];
     c->g.nc[d] = 1;
@@ -2834,6 +2962,9 @@ int ndsp_create(const int32_t* iopt, const double* dopt, int64_t npart, int32_t 
   bool ok = true;
   ok &= hipMalloc(&c->scratch_d, sizeof(double) * n * REORDER_W) == hipSuccess;
   ok &= hipMalloc(&c->scratch_i, sizeof(int) * n) == hipSuccess;
+  ok &= hipMalloc(&a.itype, sizeof(int) * n) == hipSuccess;
+  ok &= hipMalloc(&a.ireal, sizeof(int) * n) == hipSuccess;
+  ok &= hipMalloc(&a.slot_of, sizeof(int) * n) == hipSuccess;
   ok &= hipMalloc(&a.numneigh, sizeof(int) * n) == hipSuccess;
   ok &= hipMalloc(&a.nits, sizeof(int) * n) == hipSuccess;
   ok &= hipMalloc(&a.uid, sizeof(int) * n) == hipSuccess;
@@ -2852,6 +2983,7 @@ int ndsp_create(const int32_t* iopt, const double* dopt, int64_t npart, int32_t 
   ok &= hipMalloc(&a.G3, v4) == hipSuccess;
   ok &= hipMalloc(&a.G4, v4) == hipSuccess;
   ok &= hipMalloc(&a.G5, v1) == hipSuccess;
+  if (c->p.fixed_on) ok &= hipMalloc(&a.GF, v4) == hipSuccess;
   ok &= hipMalloc((void**)&a.sc, sizeof(DScal)) == hipSuccess;
   a.rcap = (int)nblk(n, 64);
   ok &= hipMalloc((void**)&a.red, sizeof(double) * RED_MAXQ * (size_t)a.rcap) == hipSuccess;
@@ -2902,8 +3034,8 @@ int ndsp_destroy(ndsp_ctx* c) {
   if (c->st) hipStreamSynchronize(c->st);
   KArgs& a = c->a;
   for (int fi = 0; fi < F_NFIELDS; fi++) if (a.f[fi]) hipFree(a.f[fi]);
-  void* ptrs[] = {c->scratch_d, c->scratch_i, a.numneigh, a.nits, a.uid, a.cellid, a.sidx, a.P4, a.dxb,
-                  a.D4, a.De, a.G0, a.G1, a.G2, a.G3, a.G4, a.G5, a.sc, (void*)a.tabf, (void*)a.tabd,
+  void* ptrs[] = {c->scratch_d, c->scratch_i, a.numneigh, a.nits, a.uid, a.itype, a.ireal, a.slot_of, a.cellid, a.sidx, a.P4, a.dxb,
+                  a.D4, a.De, a.G0, a.G1, a.G2, a.G3, a.G4, a.G5, a.GF, a.sc, (void*)a.tabf, (void*)a.tabd,
                   c->part, c->cons_out, a.cstart, a.cfill, c->bsum, c->list_a, c->list_b,
                   a.nbl, a.nbc, a.nbx, a.hcap, a.red, c->gdev};
   for (void* q : ptrs) if (q) hipFree(q);
@@ -2925,7 +3057,10 @@ int ndsp_upload(ndsp_ctx* c, const double* x, const double* vel, const double* p
   std::vector<double> buf(n);
   auto put = [&](int fi) { return hipMemcpyAsync(c->a.f[fi], buf.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->st); };
   for (int i = 0; i < n; i++) {
-    if (itype && itype[i] != 0) return set_err(NDSP_ERR_BAD_OPTION, "only gas particles (itype 0) are supported");
+    if (itype && itype[i] != 0 && itype[i] != 1)
+      return set_err(NDSP_ERR_BAD_OPTION, "only gas (itype 0) and fixed boundary (itype 1) particles are supported");
+    if (itype && itype[i] == 1 && !c->p.fixed_on)
+      return set_err(NDSP_ERR_BAD_OPTION, "fixed particles (itype 1) need a wall of fixed particles (ibound 1)");
     if (!(hh[i] > 0.0)) return set_err(NDSP_ERR_H_NONPOSITIVE, "upload: h <= 0");
   }
   for (int d = 0; d < 3; d++) {
@@ -2955,10 +3090,20 @@ int ndsp_upload(ndsp_ctx* c, const double* x, const double* vel, const double* p
   std::vector<int> ids(n);
   for (int i = 0; i < n; i++) ids[i] = i;
   HIPCHK(hipMemcpy(c->a.uid, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->a.slot_of, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  for (int i = 0; i < n; i++) ids[i] = itype ? itype[i] : 0;
+  HIPCHK(hipMemcpy(c->a.itype, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  for (int i = 0; i < n; i++) ids[i] = -1;
+  HIPCHK(hipMemcpy(c->a.ireal, ids.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  c->nfixed = 0;
+  for (int i = 0; i < n; i++) c->nfixed += itype && itype[i] == 1;
+  c->ireal_set = c->nfixed == 0;
   // derived fields and rates start at zero
   for (int fi = F_RHO; fi < F_NFIELDS; fi++)
     if (fi != F_DUSTFRAC) HIPCHK(hipMemsetAsync(c->a.f[fi], 0, sizeof(double) * n, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
+  for (int d = 0; d < 3; d++)
+    HIPCHK(hipMemcpy(c->a.f[F_BFX + d], c->a.f[F_BX + d], sizeof(double) * n, hipMemcpyDeviceToDevice));
   c->grid_valid = false;
   c->have_rates = false;
   c->nsteps = 0;
@@ -2970,7 +3115,7 @@ static const struct { const char* name; int f0; int w; } kFieldMap[] = {
     {"alpha", F_A0, 3}, {"psi", F_PSI, 1}, {"dustevol", F_EPS, 1}, {"rho", F_RHO, 1},
     {"gradh", F_GRADH, 1}, {"gradgradh", F_GRADGRADH, 1}, {"drhodt", F_DRHODT, 1},
     {"dhdt", F_DHDT, 1}, {"dens", F_DENS, 1}, {"pr", F_PR, 1}, {"spsound", F_CS, 1},
-    {"uu", F_UU, 1}, {"Bfield", F_BX, 3}, {"force", F_FX, 3}, {"dudt", F_DUDT, 1},
+    {"uu", F_UU, 1}, {"Bfield", F_BFX, 3}, {"force", F_FX, 3}, {"dudt", F_DUDT, 1},
     {"dendt", F_DENDT, 1}, {"dBevoldt", F_DBX, 3}, {"daldt", F_DA0, 3}, {"dpsidt", F_DPSIDT, 1},
     {"divB", F_DIVB, 1}, {"curlB", F_CBX, 3}, {"gradpsi", F_GPX, 3}, {"dustfrac", F_DUSTFRAC, 1},
     {"rhogas", F_RHOGAS, 1}, {"rhodust", F_RHODUST, 1}, {"ddustevoldt", F_DEPS, 1},
@@ -3076,6 +3221,7 @@ static int step_part_a(ndsp_ctx* c) {
   {
     TScope t(c, "predict");
     hipLaunchKernelGGL(k_predict, dim3(nblk(c->n)), dim3(BS), 0, c->st, c->a, c->p);
+    if (c->p.fixed_on) hipLaunchKernelGGL(k_predict_fixed, dim3(nblk(c->n)), dim3(BS), 0, c->st, c->a, c->p);
   }
   MlpState& ml = c->mlp;
   if (ml.enabled) {
@@ -3193,6 +3339,7 @@ static int ensure_key_space(ndsp_ctx* c);
 // and walls: one launch per phase covers all of them.
 struct BGroup {
   int ndim = 0, phys = 0, dust = 0, lists = 0, reflect = 0;
+  int fixed = 0;  // some member holds fixed particles
   std::vector<int> mem;
   MArgs* args = nullptr;
   // workgroup offsets per launch shape (device; nm + 1 entries each)
@@ -3246,6 +3393,7 @@ static int batch_build(ndsp_batch* b) {
       g->reflect = c->reflect;
     }
     g->mem.push_back(s);
+    g->fixed |= c->p.fixed_on;
   }
   for (auto& g : b->groups) {
     const int nm = (int)g.mem.size();
@@ -3304,6 +3452,7 @@ static int batch_part_a(ndsp_batch* b) {
   const bool fp64 = b->prec == NDSP_PREC_FP64;
   for (auto& g : b->groups) {
     BLAUNCH(km_predict, g, BGroup::S_256, BS);
+    if (g.fixed) BLAUNCH(km_predict_fixed, g, BGroup::S_256, BS);
     BLAUNCH(km_err_reset, g, BGroup::S_ONE, 64);
     BLAUNCH(km_boundary_step, g, BGroup::S_256, BS);
     BLAUNCH(km_binstats, g, BGroup::S_BIN, BS);
@@ -3362,6 +3511,7 @@ static int batch_part_a(ndsp_batch* b) {
 static int batch_part_b(ndsp_batch* b) {
   const bool fp64 = b->prec == NDSP_PREC_FP64;
   for (auto& g : b->groups) {
+    if (g.fixed) BLAUNCH(km_fixed_copy, g, BGroup::S_256, BS);
     if (fp64) BLAUNCH(km_pack_force<double>, g, BGroup::S_256, BS);
     else BLAUNCH(km_pack_force<float>, g, BGroup::S_256, BS);
     BLAUNCH(km_init_force, g, BGroup::S_ONE, 64);
@@ -3405,9 +3555,10 @@ static int batch_reorder(ndsp_batch* b, const std::vector<int>& due) {
     for (int q = 0; q < nm; q++) any |= (d[q] = due[g.mem[q]]) != 0;
     if (!any) continue;
     HIPCHK(hipMemcpyAsync(g.due, d.data(), sizeof(int) * nm, hipMemcpyHostToDevice, b->st));
-    for (int fi = 0; fi < F_NPERSIST + 3; fi++)
+    for (int fi = 0; fi < F_NPERSIST + 5; fi++)
       for (int phase = 0; phase < 2; phase++)
         BLAUNCH(km_reorder, g, BGroup::S_256, BS, (const int*)g.due, fi, phase);
+    BLAUNCH(km_slot_of, g, BGroup::S_256, BS);
     // the host vector d must outlive the copy
     HIPCHK(hipStreamSynchronize(b->st));
   }
@@ -3547,14 +3698,14 @@ struct SavedState {
 
 static int save_state(ndsp_ctx* c, SavedState& S) {
   const size_t n = (size_t)c->n;
-  if (hipMalloc(&S.buf, sizeof(double) * n * (F_NPERSIST + 2)) != hipSuccess)
+  if (hipMalloc(&S.buf, sizeof(double) * n * (F_NPERSIST + 3)) != hipSuccess)
     return set_err(NDSP_ERR_ALLOC, "state copy");
-  int* si = (int*)(S.buf + n * F_NPERSIST);  // uid, numneigh, nits
+  int* si = (int*)(S.buf + n * F_NPERSIST);  // uid, numneigh, nits, itype, ireal
+  int* const iv[5] = {c->a.uid, c->a.numneigh, c->a.nits, c->a.itype, c->a.ireal};
   for (int fi = 0; fi < F_NPERSIST; fi++)
     HIPCHK(hipMemcpyAsync(S.buf + fi * n, c->a.f[fi], sizeof(double) * n, hipMemcpyDeviceToDevice, c->st));
-  HIPCHK(hipMemcpyAsync(si, c->a.uid, sizeof(int) * n, hipMemcpyDeviceToDevice, c->st));
-  HIPCHK(hipMemcpyAsync(si + n, c->a.numneigh, sizeof(int) * n, hipMemcpyDeviceToDevice, c->st));
-  HIPCHK(hipMemcpyAsync(si + 2 * n, c->a.nits, sizeof(int) * n, hipMemcpyDeviceToDevice, c->st));
+  for (int q = 0; q < 5; q++)
+    HIPCHK(hipMemcpyAsync(si + q * n, iv[q], sizeof(int) * n, hipMemcpyDeviceToDevice, c->st));
   if (int e = sync_scalars(c, S.sc)) return e;
   S.time = c->time;
   S.dt = c->dt;
@@ -3569,9 +3720,10 @@ static int restore_state(ndsp_ctx* c, const SavedState& S) {
   const int* si = (const int*)(S.buf + n * F_NPERSIST);
   for (int fi = 0; fi < F_NPERSIST; fi++)
     HIPCHK(hipMemcpyAsync(c->a.f[fi], S.buf + fi * n, sizeof(double) * n, hipMemcpyDeviceToDevice, c->st));
-  HIPCHK(hipMemcpyAsync(c->a.uid, si, sizeof(int) * n, hipMemcpyDeviceToDevice, c->st));
-  HIPCHK(hipMemcpyAsync(c->a.numneigh, si + n, sizeof(int) * n, hipMemcpyDeviceToDevice, c->st));
-  HIPCHK(hipMemcpyAsync(c->a.nits, si + 2 * n, sizeof(int) * n, hipMemcpyDeviceToDevice, c->st));
+  int* const iv[5] = {c->a.uid, c->a.numneigh, c->a.nits, c->a.itype, c->a.ireal};
+  for (int q = 0; q < 5; q++)
+    HIPCHK(hipMemcpyAsync(iv[q], si + q * n, sizeof(int) * n, hipMemcpyDeviceToDevice, c->st));
+  hipLaunchKernelGGL(k_slot_of, dim3(nblk(c->n)), dim3(BS), 0, c->st, c->a);
   HIPCHK(hipMemcpyAsync(c->a.sc, &S.sc, sizeof(DScal), hipMemcpyHostToDevice, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   c->time = S.time;
@@ -3744,6 +3896,27 @@ int ndsp_copy_probe(ndsp_ctx* c, int64_t nbytes) {
 int ndsp_set_reorder(ndsp_ctx* c, int32_t every) {
   if (!c || every < 0) return set_err(NDSP_ERR_BAD_ARG, "set_reorder: bad arguments");
   c->reorder_every = every;
+  return 0;
+}
+
+int ndsp_set_ireal(ndsp_ctx* c, const int32_t* ireal) {
+  if (!c || !ireal) return set_err(NDSP_ERR_BAD_ARG, "set_ireal: null argument");
+  const int n = c->n;
+  std::vector<int> uid(n), it(n), ir(n);
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipMemcpy(uid.data(), c->a.uid, sizeof(int) * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(it.data(), c->a.itype, sizeof(int) * n, hipMemcpyDeviceToHost));
+  std::vector<int> type_of(n);
+  for (int s = 0; s < n; s++) type_of[uid[s]] = it[s];
+  for (int u = 0; u < n; u++) {
+    const int j = ireal[u];
+    if (j < -1 || j >= n) return set_err(NDSP_ERR_BAD_ARG, "set_ireal: index out of range");
+    if (type_of[u] == 1 && (j < 0 || type_of[j] != 0))
+      return set_err(NDSP_ERR_BAD_ARG, "set_ireal: a fixed particle must copy a gas particle");
+  }
+  for (int s = 0; s < n; s++) ir[s] = type_of[uid[s]] == 1 ? ireal[uid[s]] : -1;
+  HIPCHK(hipMemcpy(c->a.ireal, ir.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  c->ireal_set = true;
   return 0;
 }
 

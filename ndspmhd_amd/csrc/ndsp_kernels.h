@@ -16,6 +16,9 @@ enum Field {
   F_DPSIDT, F_DIVB, F_CBX, F_CBY, F_CBZ, F_GPX, F_GPY, F_GPZ, F_DEPS, F_DVX,
   F_DVY, F_DVZ, F_DEL2U,
   F_GDVX, F_GDVY, F_GDVZ,  // graddivv of the rates (hydro and dust; output only)
+  // Bfield: B at the last evaluation (a fixed particle's is its partner's),
+  // which the outputs report, as uu is en at the last evaluation (output only)
+  F_BFX, F_BFY, F_BFZ,
   F_MLPCORR,  // last step's log(h_converged / h_predictor), an MLP feature
   F_NPERSIST,  // fields above are carried by a reorder
   // step in-copies (src/stepND_leapfrog_mhd.f90:70-100)
@@ -23,6 +26,10 @@ enum Field {
   F_HIN, F_ENIN, F_A0IN, F_A1IN, F_A2IN, F_PSIIN, F_EPSIN, F_FXIN, F_FYIN, F_FZIN,
   F_DBXIN, F_DBYIN, F_DBZIN, F_DHDTIN, F_DENDTIN, F_DA0IN, F_DA1IN, F_DA2IN,
   F_DPSIDTIN, F_DEPSIN,
+  F_RHOIN,  // rho at the start of a step (held by fixed particles)
+  // a fixed particle's mass and dissipation parameters before the copy from
+  // its partner, as its periodic and mirror images carry them
+  F_MIMG, F_A0IMG, F_A1IMG, F_A2IMG,
   F_HNR0,  // h at entry of the h-rho iteration (hhin of src/iterate_density.f90:95)
   F_NFIELDS
 };
@@ -38,6 +45,7 @@ struct Phys {
   int use_list;   // neighbour lists (no reflecting walls)
   double skin;    // relative h margin of the lists
   int cell_sub;   // cells per search radius radkern*hmax (autotuned)
+  int fixed_on;   // some wall holds fixed particles (ibound 1)
 };
 
 // device scalars (one per context); doubles touched by atomics hold
@@ -86,6 +94,12 @@ struct KArgs {
   int* numneigh;
   int* nits;
   int* uid;           // caller index of each slot
+  // fixed boundary particles (ibound 1, src/set_fixedbound.f90): itype 1
+  // marks a fixed slot, ireal the caller index of the particle it copies
+  // (-1 none), slot_of the slot of each caller index
+  int* itype;
+  int* ireal;
+  int* slot_of;
   // binning
   int* cellid;        // per slot
   int* cstart;        // ncell + 1
@@ -102,6 +116,7 @@ struct KArgs {
   void* G3;           // force: Bx By Bz psi
   void* G4;           // force: h dens eps rhogas
   void* G5;           // force: rhodust
+  void* GF;           // force, images only: m a0 a1 a2 (fixed particles: before the copy)
   // kernel tables (FP32 and FP64 copies of W, dW, d2W)
   const float* tabf;  // 3 x (IKERN+1)
   const double* tabd;

@@ -107,6 +107,12 @@ module ndsp_capi
        integer(c_int32_t), intent(out) :: ncols
     end function ndsp_ev_columns
 
+    integer(c_int) function ndsp_set_ireal(ctx, ireal) bind(c, name='ndsp_set_ireal')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(in) :: ireal(*)
+    end function ndsp_set_ireal
+
     type(c_ptr) function ndsp_last_error() bind(c, name='ndsp_last_error')
        import :: c_ptr
     end function ndsp_last_error

@@ -27,7 +27,7 @@ module ndsp_host
  ! evolved state (reference layout)
  real(c_double), allocatable, save :: x(:,:), vel(:,:), Bevol(:,:), alpha(:,:)
  real(c_double), allocatable, save :: pmass(:), hh(:), en(:), psi(:), dustevol(:)
- integer(c_int32_t), allocatable, save :: itype(:)
+ integer(c_int32_t), allocatable, save :: itype(:), ireal(:)   ! ireal 0-based, -1 none
  ! derived and rates (filled by ndsp_host_sync)
  real(c_double), allocatable, save :: rho(:), force(:,:), dBevoldt(:,:), dudt(:)
  type(ndsp_stats_t), save :: stats
@@ -36,7 +36,7 @@ module ndsp_host
  integer, save :: ievfile = 13, idatfile = 12, ireadf = 15
  ! further fields for dumps (filled by ndsp_host_sync_output)
  real(c_double), allocatable, save :: dens(:), uu(:), pr(:), drhodt(:), divB(:), gradh(:)
- real(c_double), allocatable, save :: curlB(:,:), graddivv(:,:), dustfrac(:), deltav(:,:)
+ real(c_double), allocatable, save :: curlB(:,:), graddivv(:,:), dustfrac(:), deltav(:,:), Bfield(:,:)
  real(c_double), allocatable, save :: rhogas(:), rhodust(:)
 
 contains
@@ -56,8 +56,10 @@ contains
     npart = n
     ndim = nd
     allocate(x(nd,n), vel(3,n), Bevol(3,n), alpha(3,n), pmass(n), hh(n), en(n), psi(n), dustevol(n), itype(n))
+    allocate(ireal(n))
+    ireal = -1
     allocate(rho(n), force(3,n), dBevoldt(3,n), dudt(n))
-    allocate(dens(n), uu(n), pr(n), drhodt(n), divB(n), gradh(n), curlB(3,n), graddivv(3,n))
+    allocate(dens(n), uu(n), pr(n), drhodt(n), divB(n), gradh(n), curlB(3,n), graddivv(3,n), Bfield(3,n))
     allocate(dustfrac(n), deltav(3,n), rhogas(n), rhodust(n))
     itype = 0
     psi = 0.
@@ -72,6 +74,7 @@ contains
                          'ndsp_create')
     call ndsp_host_check(ndsp_upload(ctx, x, vel, pmass, hh, en, Bevol, alpha, psi, dustevol, itype), &
                          'ndsp_upload')
+    if (any(itype == 1)) call ndsp_host_check(ndsp_set_ireal(ctx, ireal), 'ndsp_set_ireal')
     dt = dopt(24)
     time = 0.
     call ndsp_host_check(ndsp_set_dt(ctx, dt, time), 'ndsp_set_dt')
@@ -114,6 +117,7 @@ contains
     call get_field('divB', divB, npart)
     call get_field('gradh', gradh, npart)
     call get_field('curlB', curlB, 3*npart)
+    call get_field('Bfield', Bfield, 3*npart)
     call get_field('graddivv', graddivv, 3*npart)
     call get_field('dustfrac', dustfrac, npart)
     call get_field('deltav', deltav, 3*npart)
@@ -279,7 +283,7 @@ contains
           write(idatfile) alpha(i,1:nprint)
        enddo
        do i = 1, ndimV
-          write(idatfile) Bevol(i,1:nprint)   ! Bfield = Bevol for imhd 11
+          write(idatfile) Bfield(i,1:nprint)
        enddo
        write(idatfile) psi(1:nprint)
        write(idatfile) pr(1:nprint)

@@ -8,9 +8,9 @@
 !   ndsp_run <input.bin> <output.bin> <nsteps> [tmax] [precision] [rootname]
 !            [nout] [restart.dat]
 !
-! input.bin has the layout of the reference harness input (options iopt/dopt
-! and the particle state; tests/golden/refio.py).  With a rootname the run
-! writes rootname.ev and the dumps rootname_00000.dat (initial state),
+! input.bin has the layout of the reference harness input (options iopt/dopt,
+! the particle state and the partners of fixed particles;
+! tests/golden/refio.py).  With a rootname the run writes rootname.ev and the dumps rootname_00000.dat (initial state),
 ! then one every nout steps (nout > 0) and one after the last step, named as
 ! the reference's output does (src/outputND_mhd.f90:54).  restart.dat
 ! replaces the state and time of input.bin by those of a dump.  Every step
@@ -53,7 +53,7 @@ program ndsp_run
  read(21) iopt
  read(21) dopt
  call ndsp_host_alloc(np, nd)
- read(21) x, vel, pmass, hh, en, Bevol, alpha, psi, dustevol, itype
+ read(21) x, vel, pmass, hh, en, Bevol, alpha, psi, dustevol, itype, ireal
  close(21)
  tfile = 0.
  if (len_trim(restart) > 0) call read_dump(trim(restart), tfile)

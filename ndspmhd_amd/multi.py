@@ -25,6 +25,7 @@ import numpy as np
 
 from . import infile as inf
 from .options import empty_state
+from .setups import fixed_ends_1d
 
 # the seven MHD shock tubes of multi/multi_shock.f90:57-220: psep, gamma,
 # tmax, tout, iener override (None: the base file's), polyk, and the states
@@ -107,7 +108,9 @@ def shock_setup_1d(values: dict, shk, dsmooth: float = 20.0):
     """(opts, state) of src/setup_shock1D_mhd.f90:45-175: x in [-0.5, 0.5],
     massp = rho_R psep, particles placed at x(i) = x(i-2) + 2 massp/rho(i-1)
     from the left wall, states joined by a logistic profile in
-    delta = 2 (x - xc)/psep over |delta| <= dsmooth; reflecting walls"""
+    delta = 2 (x - xc)/psep over |delta| <= dsmooth; the first and last six
+    particles held fixed (ibound 1, nbpts 6, src/setup_shock1D_mhd.f90:47-48,
+    src/set_fixedbound.f90:38-53)"""
     (rl, rr, pl, pr_, vxl, vxr, vyl, vyr, vzl, vzr, bx, byl, byr, bzl, bzr) = shk
     psep, gamma, imhd = values["psep"], values["gamma"], values["imhd"]
     xmin, xmax = -0.5, 0.5
@@ -150,11 +153,11 @@ def shock_setup_1d(values: dict, shk, dsmooth: float = 20.0):
     st["en"][:] = uu[:n]
     st["vel"][:] = np.array(vel[:n])
     st["Bevol"][:] = np.array(B[:n])
-    o = inf.to_options(values, 1, ibound=(2, 0, 0), xmin=(xmin, 0.0, 0.0), xmax=(xmax, 0.0, 0.0))
+    o = inf.to_options(values, 1, ibound=(1, 0, 0), xmin=(xmin, 0.0, 0.0), xmax=(xmax, 0.0, 0.0))
     st["alpha"][:, 0] = o.alphamin
     st["alpha"][:, 1] = o.alphaumin
     st["alpha"][:, 2] = o.alphaBmin
-    return o, st
+    return o, fixed_ends_1d(st, 6)
 
 
 @dataclass

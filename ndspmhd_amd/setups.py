@@ -125,13 +125,55 @@ def dusty_box(nside: int, ndim: int = 2, seed: int = 3, amp: float = 1.0e-2,
     return o, st
 
 
+def fixed_ends_1d(st: dict, nbpts: int = 6) -> dict:
+    """the first and last nbpts particles of a 1D row (in x order) become
+    fixed boundary particles copying the first interior particle at their
+    end, with its h (src/set_fixedbound.f90:32-48)"""
+    n = st["pmass"].shape[0]
+    order = np.argsort(st["x"][:, 0], kind="stable")
+    st["itype"] = np.zeros(n, dtype=np.int32)
+    st["ireal"] = np.full(n, -1, dtype=np.int32)
+    left, right = order[:nbpts], order[n - nbpts:]
+    st["itype"][left] = 1
+    st["itype"][right] = 1
+    st["ireal"][left] = order[nbpts]
+    st["ireal"][right] = order[n - nbpts - 1]
+    fixed = st["itype"] == 1
+    st["hh"][fixed] = st["hh"][st["ireal"][fixed]]
+    return st
+
+
+def fixed_near_walls(st: dict, xmin: float, xmax: float, wl: float, wr: float) -> dict:
+    """particles closer than wl to the left x wall or wr to the right one
+    become fixed boundary particles (src/setup_shockND.f90:334-343), each
+    copying the closest other gas particle, the first in index order on a tie
+    (src/set_fixedbound.f90:87-106)"""
+    x = np.asarray(st["x"], dtype=np.float64)
+    n = x.shape[0]
+    fixed = (x[:, 0] < xmin + wl) | (x[:, 0] > xmax - wr)
+    st["itype"] = np.where(fixed, 1, 0).astype(np.int32)
+    st["ireal"] = np.full(n, -1, dtype=np.int32)
+    gas = np.flatnonzero(~fixed)
+    xg = x[gas]
+    for i in np.flatnonzero(fixed):
+        dx = x[i] - xg
+        rr = dx[:, 0] * dx[:, 0]
+        for d in range(1, x.shape[1]):
+            rr = rr + dx[:, d] * dx[:, d]
+        st["ireal"][i] = gas[int(np.argmin(rr))]
+    return st
+
+
 def brio_wu_1d(npart_target: int = 800, hfact: float = 1.2,
-               states=None, gamma: float = 2.0, opts: Options | None = None):
+               states=None, gamma: float = 2.0, opts: Options | None = None,
+               boundary: str = "reflect", nbpts: int = 6):
     """1D MHD shock tube (Brio & Wu) with equal-mass particles on two
     lattices: x in [-0.5, 0.5], left rho=1 p=1 By=1, right rho=0.125 p=0.1
-    By=-1, Bx=0.75, v=0, gamma=2, reflecting walls
-    (src/setup_shockND.f90:200-330; states as multi/multi_shock.f90:66-82).
-    `states` = (rhoL, rhoR, pL, pR, ByL, ByR, Bx) overrides the defaults."""
+    By=-1, Bx=0.75, v=0, gamma=2 (src/setup_shockND.f90:200-330; states as
+    multi/multi_shock.f90:66-82).  `states` = (rhoL, rhoR, pL, pR, ByL, ByR,
+    Bx) overrides the defaults.  boundary "reflect": reflecting walls
+    (ibound 2); "fixed": the reference's walls of nbpts fixed particles at
+    each end (ibound 1, src/setup_shock1D_mhd.f90:47-48)."""
     rhoL, rhoR, pL, pR, ByL, ByR, Bx = states or (1.0, 0.125, 1.0, 0.1, 1.0, -1.0, 0.75)
     # equal masses: nL/nR = rhoL/rhoR over equal lengths
     nL = int(round(npart_target * rhoL / (rhoL + rhoR)))
@@ -154,22 +196,27 @@ def brio_wu_1d(npart_target: int = 800, hfact: float = 1.2,
     o = opts or Options()
     o = o.replace(ndim=1, ikernel=0, iener=2, imhd=11, imagforce=2,
                   idivbzero=2, iav=2, iavlim=(2, 1, 0), idust=0,
-                  ibound=(2, 0, 0), hfact=hfact, gamma=gamma,
+                  ibound=(1 if boundary == "fixed" else 2, 0, 0), hfact=hfact, gamma=gamma,
                   xmin=(-0.5, 0.0, 0.0), xmax=(0.5, 0.0, 0.0), psep=psepL)
     st["alpha"][:, 0] = o.alphamin
     st["alpha"][:, 1] = o.alphaumin
     st["alpha"][:, 2] = o.alphaBmin
+    if boundary == "fixed":
+        st = fixed_ends_1d(st, nbpts)
     return o, st
 
 
 def shock_tube_2d(npart_target: int = 2000, hfact: float = 1.2, states=None,
-                  gamma: float = 2.0, width_rows: int = 24, opts: Options | None = None):
+                  gamma: float = 2.0, width_rows: int = 24, opts: Options | None = None,
+                  boundary: str = "fixed"):
     """2D MHD shock tube: x in [-0.5, 0.5] with the Brio-Wu states of
     brio_wu_1d, a periodic y extent of `width_rows` left-state lattice
     spacings (src/setup_shockND.f90:211-215), equal particle masses (the
-    right lattice is sparser by rhoL/rhoR in area).  The x walls reflect
-    (ibound 2); the reference's 2D tube holds the x-boundary particles fixed
-    (ibound 1), which this library does not implement."""
+    right lattice is sparser by rhoL/rhoR in area).  As the reference's
+    tube in more than one dimension (src/setup_shockND.f90:201-202,334-343),
+    the particles within 1.1 radkern hfact psep of an x wall are fixed
+    boundary particles (ibound 1) copying their closest gas particle;
+    boundary "reflect" gives reflecting x walls (ibound 2) instead."""
     rhoL, rhoR, pL, pR, ByL, ByR, Bx = states or (1.0, 0.125, 1.0, 0.1, 1.0, -1.0, 0.75)
     # left lattice: nxL x ny over [-0.5, 0] x [0, Ly]
     area_frac = rhoL / (rhoL + rhoR)
@@ -204,11 +251,14 @@ def shock_tube_2d(npart_target: int = 2000, hfact: float = 1.2, states=None,
     o = opts or Options()
     o = o.replace(ndim=2, ikernel=0, iener=2, imhd=11, imagforce=2,
                   idivbzero=2, iav=2, iavlim=(2, 1, 0), idust=0,
-                  ibound=(2, 3, 0), hfact=hfact, gamma=gamma,
+                  ibound=(1 if boundary == "fixed" else 2, 3, 0), hfact=hfact, gamma=gamma,
                   xmin=(-0.5, 0.0, 0.0), xmax=(0.5, Ly, 0.0), psep=psepL)
     st["alpha"][:, 0] = o.alphamin
     st["alpha"][:, 1] = o.alphaumin
     st["alpha"][:, 2] = o.alphaBmin
+    if boundary == "fixed":
+        radkern = 2.0  # cubic spline
+        st = fixed_near_walls(st, -0.5, 0.5, 1.1 * radkern * hfact * psepL, 1.1 * radkern * hfact * psepRx)
     return o, st
 
 

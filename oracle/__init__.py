@@ -44,6 +44,7 @@ def lib():
         L.orc_scalar.restype = ctypes.c_double
         L.orc_scalar.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
         L.orc_set_scalar.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_double]
+        L.orc_set_ireal.argtypes = [ctypes.c_void_p, ip]
         L.orc_neighbours.restype = ctypes.c_int64
         L.orc_neighbours.argtypes = [ctypes.c_void_p, ctypes.c_int, ip, ctypes.c_int64]
         L.orc_conserved.argtypes = [ctypes.c_void_p, dp]
@@ -91,6 +92,9 @@ class Oracle:
         it = np.ascontiguousarray(state["itype"], dtype=np.int32)
         L.orc_set_state(self.h, *[_dp(a[k]) for k in ("x", "vel", "pmass", "hh", "en",
                                                        "Bevol", "alpha", "psi", "dustevol")], _ip(it))
+        if "ireal" in state:
+            ir = np.ascontiguousarray(state["ireal"], dtype=np.int32)
+            self._check(L.orc_set_ireal(self.h, _ip(ir)), "set_ireal")
 
     def __del__(self):
         h = getattr(self, "h", None)

@@ -237,6 +237,13 @@ static inline void kinterp(const orc_sim *s, double q2, double *w, double *grw,
 /* boundaries and ghosts                                                     */
 /* ------------------------------------------------------------------------ */
 
+/* fixed boundary particles in use (any ibound = 1, set_fixedbound.f90) */
+static int anyfixed(const orc_sim *s) {
+  for (int d = 0; d < s->ndim; d++)
+    if (s->ibound[d] == 1) return 1;
+  return 0;
+}
+
 /* copy_particle, copy_particle.f90:28-115 (the fields this oracle keeps) */
 static void copy_particle(orc_sim *s, int i, int j) {
   for (int k = 0; k < NDUNIQUE; k++) {
@@ -741,6 +748,16 @@ static int iterate_density(orc_sim *s) {
         s->dhdt[i] = dhdrhoi * s->drhodt[i];
       }
     }
+    if (anyfixed(s)) {
+      /* write over boundary particles, iterate_density.f90:310-326 */
+      for (int i = 0; i < npart; i++) {
+        if (s->itype[i] != ITYPEBND) continue;
+        int j = s->ireal[i];
+        if (j < 0) continue;
+        s->rho[i] = s->rho[j]; s->drhodt[i] = s->drhodt[j]; s->dhdt[i] = s->dhdt[j];
+        s->hh[i] = s->hh[j]; s->gradh[i] = s->gradh[j];
+      }
+    }
     if (anyghost) {
       for (int i = npart; i < s->ntotal; i++) {
         int j = s->ireal[i];
@@ -794,6 +811,15 @@ static int cons2prim(orc_sim *s) {
         s->pr[i] = gamma1 * s->uu[i] * rhoe[i];
         s->spsound[i] = sqrt(s->gamma * s->pr[i] / rhoe[i]);
       }
+    }
+  }
+  if (anyfixed(s)) {
+    /* make fixed particles exact replicas of their closest particle,
+       conservative2primitive.f90:425-432 */
+    for (int i = 0; i < npart; i++) {
+      if (s->itype[i] != ITYPEBND && s->itype[i] != ITYPEBNDDUST) continue;
+      int j = s->ireal[i];
+      if (j >= 0) copy_particle(s, i, j);
     }
   }
   int anyghost = 0, allper = 1;
@@ -1226,7 +1252,9 @@ static int get_rates(orc_sim *s) {
       s->dpsidt[i] = 0.;
   }
   if (fhmax > 0.) s->dtforce = fmin(s->dtforce, sqrt(1. / fhmax));
-  for (int i = npart; i < ntotal; i++) {
+  /* rates zero on ghosts and fixed particles, ratesND_mhd.f90:949-966 */
+  for (int i = 0; i < ntotal; i++) {
+    if (i < npart && s->itype[i] != ITYPEBND && s->itype[i] != ITYPEBNDDUST) continue;
     for (int c = 0; c < 3; c++) {
       s->force[i * 3 + c] = 0.; s->dBevoldt[i * 3 + c] = 0.; s->daldt[i * 3 + c] = 0.;
       s->gradpsi[i * 3 + c] = 0.; s->curlB[i * 3 + c] = 0.;
@@ -1263,7 +1291,7 @@ int orc_derivs(orc_sim *s) {
   return 0;
 }
 
-/* step, stepND_leapfrog_mhd.f90:31-275 (no fixed particles, damp = 0) */
+/* step, stepND_leapfrog_mhd.f90:31-275 (damp = 0) */
 int orc_step(orc_sim *s) {
   const int npart = s->npart, ndim = s->ndim;
   const double dt = s->dt, hdt = 0.5 * dt;
@@ -1290,8 +1318,22 @@ int orc_step(orc_sim *s) {
       s->ddustevoldtin[i] = s->ddustevoldt[i];
     }
   }
-  /* predictor, :107-165 */
+  /* predictor, :107-165; fixed particles (:110-143) move with the velocity
+     and force of the particle they copy and hold everything else */
   for (int i = 0; i < npart; i++) {
+    if (s->itype[i] == ITYPEBND || s->itype[i] == ITYPEBNDDUST) {
+      const int j = s->ireal[i] >= 0 ? s->ireal[i] : i;
+      for (int d = 0; d < ndim; d++)
+        s->x[i * 3 + d] = s->xin[i * 3 + d] + dt * s->velin[j * 3 + d] + 0.5 * dt * dt * s->forcein[j * 3 + d];
+      for (int c = 0; c < 3; c++) s->Bevol[i * 3 + c] = s->Bevolin[i * 3 + c];
+      s->rho[i] = s->rhoin[i];
+      s->hh[i] = s->hhin[i];
+      s->en[i] = s->enin[i];
+      for (int c = 0; c < 3; c++) s->alpha[i * 3 + c] = s->alphain[i * 3 + c];
+      s->psi[i] = s->psiin[i];
+      if (s->onef_dust) s->dustevol[i] = s->dustevolin[i];
+      continue;
+    }
     for (int d = 0; d < ndim; d++)
       s->x[i * 3 + d] = s->xin[i * 3 + d] + dt * s->velin[i * 3 + d] + 0.5 * dt * dt * s->forcein[i * 3 + d];
     for (int c = 0; c < 3; c++)
@@ -1309,8 +1351,22 @@ int orc_step(orc_sim *s) {
   }
   int e = orc_derivs(s);
   if (e) return e;
-  /* corrector, :169-218 */
+  /* corrector, :169-218; fixed particles (:172-186) get back their own
+     starting velocity and held values */
   for (int i = 0; i < npart; i++) {
+    if (s->itype[i] == ITYPEBND || s->itype[i] == ITYPEBNDDUST) {
+      for (int c = 0; c < 3; c++) {
+        s->vel[i * 3 + c] = s->velin[i * 3 + c];
+        s->Bevol[i * 3 + c] = s->Bevolin[i * 3 + c];
+        s->alpha[i * 3 + c] = s->alphain[i * 3 + c];
+      }
+      s->rho[i] = s->rhoin[i];
+      s->hh[i] = s->hhin[i];
+      s->en[i] = s->enin[i];
+      s->psi[i] = s->psiin[i];
+      if (s->onef_dust) s->dustevol[i] = s->dustevolin[i];
+      continue;
+    }
     for (int c = 0; c < 3; c++)
       s->vel[i * 3 + c] = (s->velin[i * 3 + c] + hdt * (s->force[i * 3 + c] + s->forcein[i * 3 + c])) / (1. + s->damp);
     if (s->imhd != 0)
@@ -1373,7 +1429,7 @@ orc_sim *orc_new(const int32_t *iopt, const double *dopt, int32_t npart, char *e
   int nrefl = 0;
   for (int d = 0; d < s->ndim; d++) {
     int b = s->ibound[d];
-    if (!(b == 0 || b == 2 || b == 3)) REJECT("ibound must be 0, 2 or 3");
+    if (!(b == 0 || b == 1 || b == 2 || b == 3)) REJECT("ibound must be 0, 1, 2 or 3");
     if (b == 2) nrefl++;
   }
   if (nrefl > 1) REJECT("reflecting walls in at most one dimension");
@@ -1438,7 +1494,18 @@ int orc_set_state(orc_sim *s, const double *x, const double *vel, const double *
     s->dustevol[i] = s->onef_dust ? dustevol[i] : 0.;
     s->dustfrac[i] = s->dustevol[i];
     s->itype[i] = itype[i];
+    s->ireal[i] = -1;
     s->rho[i] = 0.;
+  }
+  return 0;
+}
+
+/* the particle each fixed particle copies (0-based, -1 none), as
+   set_fixedbound.f90:32-48 sets ireal */
+int orc_set_ireal(orc_sim *s, const int32_t *ireal) {
+  for (int i = 0; i < s->npart; i++) {
+    if (ireal[i] >= s->npart || (ireal[i] >= 0 && s->itype[ireal[i]] != ITYPEGAS)) return fail(s, 8, "bad ireal");
+    s->ireal[i] = ireal[i];
   }
   return 0;
 }

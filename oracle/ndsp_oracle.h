@@ -52,6 +52,7 @@ int orc_get(const orc_sim *s, const char *name, double *out);
  * itsdensity ntotal stressmax time; NaN for an unknown name */
 double orc_scalar(const orc_sim *s, const char *name);
 int orc_set_scalar(orc_sim *s, const char *name, double v);
+int orc_set_ireal(orc_sim *s, const int32_t *ireal);
 
 /* message of the last failure */
 const char *orc_message(const orc_sim *s);

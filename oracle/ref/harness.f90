@@ -47,7 +47,7 @@ program ndsp_ref_harness
  real    :: dopt(maxopt)
  integer :: magic,mode,np,ndimin,ierr,ierr1,ierr2,i,j,n,k,nneigh,icode,ip
  integer :: npg,nps
- integer, allocatable :: listneigh(:),pg(:,:),ps(:,:)
+ integer, allocatable :: listneigh(:),pg(:,:),ps(:,:),irin(:)
  real, allocatable :: dustin(:),zeros(:),zeros3(:,:)
  real :: rij2,q2i,q2j,hi1,hi21,hj1,etot,momtot
  integer(8) :: count0,count1,count_rate
@@ -152,6 +152,11 @@ program ndsp_ref_harness
  read(21) dustin(1:np)
  if (onef_dust) dustevol(1,1:np) = dustin(1:np)
  read(21) itype(1:np)
+!--partners of fixed particles (0-based, -1 none) -> ireal (1-based, 0 none),
+!  as set_fixedbound (src/set_fixedbound.f90:32-48) would set them
+ allocate(irin(np))
+ read(21) irin(1:np)
+ ireal(1:np) = irin(1:np) + 1
  close(21)
  rho(1:np) = 0.
  uu(1:np) = en(1:np)

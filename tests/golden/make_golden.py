@@ -41,6 +41,20 @@ def perturb_mhd(st, seed):
     return st
 
 
+def fixed_walls(st, nside):
+    """walls of fixed particles at both x ends of a lattice box (x fastest):
+    the two outer layers copy the first interior layer of their (y, z) row,
+    with its h (set_fixedbound.f90:32-48 does this for 1D rows)"""
+    n = st["pmass"].shape[0]
+    ix = np.arange(n) % nside
+    row = np.arange(n) - ix
+    st["itype"] = np.where((ix < 2) | (ix >= nside - 2), 1, 0).astype(np.int32)
+    st["ireal"] = np.where(ix < 2, row + 2, np.where(ix >= nside - 2, row + nside - 3, -1)).astype(np.int32)
+    fixed = st["itype"] == 1
+    st["hh"][fixed] = st["hh"][st["ireal"][fixed]]
+    return st
+
+
 def cases():
     o, st = periodic_mhd_box(8, seed=11, vel_seed=12)
     st = perturb_mhd(st, 13)
@@ -55,6 +69,9 @@ def cases():
     o, st = periodic_mhd_box(8, seed=41, vel_seed=42, jitter=0.2)
     st = perturb_mhd(st, 43)
     yield "hydro3d_nr", o.replace(imhd=0), st
+    o, st = periodic_mhd_box(8, seed=51, vel_seed=52, jitter=0.15)
+    st = perturb_mhd(st, 53)
+    yield "mhd3d_fixedwall", o.replace(ibound=(1, 3, 3)), fixed_walls(st, 8)
     o, st = dusty_box(8, ndim=3, seed=31)
     rng = np.random.default_rng(32)
     st["vel"][:] = 0.05 * rng.standard_normal(st["vel"].shape)

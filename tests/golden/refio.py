@@ -5,7 +5,7 @@ Input file (little-endian stream):
   int32  iopt[32]      (slots: ndspmhd_amd.options.IOPT)
   float64 dopt[32]     (slots: ndspmhd_amd.options.DOPT)
   float64 x(ndim,N) vel(3,N) pmass(N) hh(N) en(N) Bevol(3,N) alpha(3,N) psi(N) dustevol(N)
-  int32   itype(N)
+  int32   itype(N) ireal(N)  (partner of each fixed particle, 0-based; -1 none)
 Output file:
   int32  npart, ntotal, itsdensity, 0
   float64 dtcourant, dtforce, dtdrag, dtav, vsig2max, hhmax
@@ -41,6 +41,7 @@ def write_input(path, opts, st, mode=1):
         for k in ("x", "vel", "pmass", "hh", "en", "Bevol", "alpha", "psi", "dustevol"):
             np.ascontiguousarray(st[k], dtype="<f8").tofile(f)
         np.ascontiguousarray(st["itype"], dtype="<i4").tofile(f)
+        np.ascontiguousarray(st.get("ireal", np.full(n, -1)), dtype="<i4").tofile(f)
 
 
 def read_output(path, ndim, mode=1):

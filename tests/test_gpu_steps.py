@@ -31,7 +31,12 @@ def test_mhd_box_steps(prec, tol):
     st["Bevol"] += 0.03 * rng.standard_normal(st["Bevol"].shape)
     print("mhd box 10^3 steps prec", prec, flush=True)
     sim, orc = _run(o, st, 5, prec)
-    for k in ("x", "vel", "Bevol", "en", "hh", "alpha", "psi", "rho", "force", "dBevoldt"):
+    # Bfield and uu are B and en at the last evaluation (the predictor's),
+    # as the reference's .ev sums and dumps report them
+    ca, cb = sim.conserved(), orc.conserved()
+    for k in ("ekin", "etherm", "emag", "etot"):
+        assert ca[k] == pytest.approx(cb[k], rel=tol), (k, ca[k], cb[k])
+    for k in ("x", "vel", "Bevol", "Bfield", "en", "uu", "hh", "alpha", "psi", "rho", "force", "dBevoldt"):
         a, b = sim.get(k), orc.get(k)
         err = np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
         print(k, err, flush=True)
@@ -65,13 +70,17 @@ def test_brio_wu_1d_steps(prec, tol):
         assert err <= (tol if k != "x" else tol * 1e-2), (k, err)
 
 
+@pytest.mark.parametrize("boundary", ["reflect", "fixed"])
 @pytest.mark.parametrize("prec,tol", [(1, 1e-9), (0, 1e-4)])
-def test_shock_tube_2d_steps(prec, tol):
-    """reflecting x walls with a periodic y: edge images follow the
-    reference's ghost velocities (src/ghostND_mhd.f90:281-282)"""
+def test_shock_tube_2d_steps(prec, tol, boundary):
+    """x walls with a periodic y.  Reflecting: edge images follow the
+    reference's ghost velocities (src/ghostND_mhd.f90:281-282).  Fixed: the
+    periodic images of fixed particles keep the mass and alpha the fixed
+    particle had before its partner's were copied in (the reference's ghosts
+    are made before that copy)."""
     from ndspmhd_amd.setups import shock_tube_2d
-    o, st = shock_tube_2d(600, hfact=1.2)
-    print("2d shock tube steps prec", prec, flush=True)
+    o, st = shock_tube_2d(600, hfact=1.2, boundary=boundary)
+    print("2d shock tube steps prec", prec, boundary, flush=True)
     sim, orc = _run(o, st, 6, prec)
     bad = []
     for k in ("x", "vel", "Bevol", "en", "hh", "rho", "force", "dBevoldt"):
@@ -81,3 +90,34 @@ def test_shock_tube_2d_steps(prec, tol):
         if err > (tol if k != "x" else tol * 1e-2):
             bad.append((k, err))
     assert not bad, bad
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-9), (0, 1e-4)])
+def test_brio_wu_1d_fixed_walls_steps(prec, tol):
+    """the reference's shock tube with walls of fixed particles (ibound 1):
+    GPU against the oracle over 10 steps, plus what the fixed particles must
+    do whatever the arithmetic: keep their velocity, take their partner's
+    state after every evaluation and carry zero rates"""
+    from ndspmhd_amd.setups import brio_wu_1d
+    o, st = brio_wu_1d(400, boundary="fixed")
+    print("brio-wu 1d fixed walls steps prec", prec, flush=True)
+    sim, orc = _run(o, st, 10, prec)
+    # Bfield of a fixed particle is its partner's at the last evaluation while
+    # its Bevol is held: the energies (evwrite sums Bfield) see the former
+    ca, cb = sim.conserved(), orc.conserved()
+    for k in ("ekin", "etherm", "emag", "etot"):
+        assert ca[k] == pytest.approx(cb[k], rel=tol), (k, ca[k], cb[k])
+    for k in ("x", "vel", "Bevol", "Bfield", "en", "hh", "rho", "force"):
+        a, b = sim.get(k), orc.get(k)
+        err = np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+        print(k, err, flush=True)
+        assert err <= (tol if k != "x" else tol * 1e-2), (k, err)
+    fixed = st["itype"] == 1
+    part = st["ireal"][fixed]
+    np.testing.assert_array_equal(sim.get("vel")[fixed], st["vel"][fixed])
+    for k in ("force", "dBevoldt", "dudt", "daldt", "dpsidt", "divB"):
+        assert np.all(sim.get(k)[fixed] == 0.0), k
+    # replicas of their partners right after an evaluation
+    sim.derivs()
+    for k in ("rho", "hh", "pr", "en"):
+        np.testing.assert_array_equal(sim.get(k)[fixed], sim.get(k)[part], err_msg=k)

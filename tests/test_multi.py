@@ -50,5 +50,11 @@ def test_shock_setup(i, tmp_path):
     assert dr == pytest.approx(2 * m[0] / shk[1], rel=1e-9)
     # total mass matches the states over the two halves
     assert m.sum() == pytest.approx(0.5 * (shk[0] + shk[1]), rel=0.02)
-    assert o.ibound[0] == 2 and o.ndim == 1 and o.gamma == v["gamma"]
+    assert o.ibound[0] == 1 and o.ndim == 1 and o.gamma == v["gamma"]
+    # the first and last six particles are fixed, copying the seventh from
+    # their end, with its h
+    n = x.shape[0]
+    assert np.all(st["itype"][:6] == 1) and np.all(st["itype"][-6:] == 1) and st["itype"][6:-6].sum() == 0
+    assert np.all(st["ireal"][:6] == 6) and np.all(st["ireal"][-6:] == n - 7)
+    assert np.all(st["hh"][:6] == st["hh"][6]) and np.all(st["hh"][-6:] == st["hh"][n - 7])
     assert np.all(st["vel"][:5] == shk[4:10:2]) and np.all(st["vel"][-5:] == shk[5:10:2])
