@@ -34,6 +34,8 @@ HBM_PEAK_GBS = 8000.0
 # (8 B per real, each field read once and each output written once)
 BYTES_FORCE = 280        # 20 reals in, 15 reals out
 BYTES_DENSITY = 108      # per active particle and h-rho pass
+BYTES_NBLIST = 40        # neighbour lists: x, h in; count and h cap out
+BYTES_NBLIST_ENTRY = 4   # per list entry, at least numneigh of them
 FLOPS_FORCE_PAIR = 200   # FP32 flops per ordered pair evaluation (estimate, see DESIGN)
 
 
@@ -52,12 +54,14 @@ def setup(nside):
     return _SETUPS[nside]
 
 
-def run_gpu(nside, steps, warmup, prec, timing=True, graph=False):
+def run_gpu(nside, steps, warmup, prec, timing=True, graph=False, tune=True):
     import ndspmhd_amd as nd
     o, st = setup(nside)
     sim = nd.Simulation(o, st, precision=prec)
-    sim.set_graph(graph)
     sim.derivs()
+    # tile shapes chosen by timing derivs on this workload (state unchanged)
+    shapes = sim.autotune(reps=2) if tune else None
+    sim.set_graph(graph)
     for _ in range(warmup):
         sim.step()
     sim.set_timing(timing)
@@ -73,7 +77,9 @@ def run_gpu(nside, steps, warmup, prec, timing=True, graph=False):
     t1 = time.perf_counter()
     kt = sim.kernel_times() if timing else {}
     cons1 = sim.conserved()
-    return dict(sim=sim, opts=o, wall=t1 - t0, pairs_density=pd, pairs_force=pf,
+    numneigh = float(np.mean(sim.get("numneigh")))
+    return dict(sim=sim, opts=o, wall=t1 - t0, pairs_density=pd, pairs_force=pf, shapes=shapes,
+                numneigh_mean=numneigh,
                 ncalc_mean=float(np.mean(nits)), kernel_ms=kt, n=nside ** 3,
                 cons=(cons0, cons1, float(np.sum(st["pmass"]))))
 
@@ -279,14 +285,15 @@ def main():
     ms_step = 1e3 * r["wall"] / args.steps
     kt = r["kernel_ms"]
     log("[bench] %.3f ms/step, %.3e pairs/s, kernels %s" % (ms_step, value, kt))
-    dom = max(kt, key=kt.get) if kt else "force"
+    pair_kernels = {k: kt[k] for k in ("nblist", "density", "force") if k in kt}
+    dom = max(pair_kernels, key=pair_kernels.get) if pair_kernels else "force"
     force_ms = kt.get("force", float("nan"))
-    dens_ms = kt.get("density", float("nan"))
     # algorithmic bytes per launch of the dominant kernel
     if dom == "density":
         alg_bytes = BYTES_DENSITY * n * r["ncalc_mean"]
+    elif dom == "nblist":
+        alg_bytes = (BYTES_NBLIST + BYTES_NBLIST_ENTRY * r["numneigh_mean"]) * n
     else:
-        dom = "force"
         alg_bytes = BYTES_FORCE * n
     dom_ms = kt.get(dom, float("nan"))
     achieved = alg_bytes / (dom_ms * 1e-3) / 1e9
@@ -310,10 +317,12 @@ def main():
                                "gamma 5/3, v_rms Mach 1 P(k)~k^-4, B=(0.1,0,0)" % (args.nside, n),
                    "particles": n, "steps": args.steps},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile("k_" + dom),
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile({"density": "k_nr_pass"}.get(dom, "k_" + dom)),
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": dom_ms,
                      "force_fp32_tflops": flops / (force_ms * 1e-3) / 1e12 if force_ms == force_ms else None},
         "kernel_ms": kt,
+        "tile_shapes": r["shapes"],
+        "numneigh_mean": r["numneigh_mean"],
         "pairs_per_step": {"density": r["pairs_density"] / args.steps, "force": pairs_force_step},
         "ncalc_mean": r["ncalc_mean"],
         "conservation": conservation(*r["cons"]),

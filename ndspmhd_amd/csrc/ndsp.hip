@@ -248,16 +248,37 @@ __device__ __forceinline__ int cell_coord(const Grid& g, int d, double x) {
   return c;
 }
 
+// Consecutive lanes of a wave holding the same key (-1: none) form a run; the
+// run's first lane adds the run's length to ctr[key] once, and each lane gets
+// the old value plus its place in the run.  Slots are in cell order after a
+// reorder, so a wave meets few cells: few atomics, and the particles of a
+// cell land in slot order within each run.
+__device__ __forceinline__ int run_atomic_add(int* __restrict__ ctr, int key) {
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(key, 1);
+  const bool head = lane == 0 || prev != key;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const int myhead = 63 - __clzll(heads & upto);
+  const unsigned long long after = heads & ~upto;
+  const int next = after ? __ffsll((long long)after) - 1 : 64;
+  int base = 0;
+  if (head && key >= 0) base = atomicAdd(&ctr[key], next - lane);
+  return __shfl(base, myhead) + (lane - myhead);
+}
+
 __device__ __forceinline__ void b_cellid(KArgs a, const Grid* __restrict__ gp, VL vl) {
-  int i = vl.b * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
-  const Grid g = *gp;
-  int cx = cell_coord(g, 0, a.f[F_X][i]);
-  int cy = g.ndim >= 2 ? cell_coord(g, 1, a.f[F_Y][i]) : 0;
-  int cz = g.ndim >= 3 ? cell_coord(g, 2, a.f[F_Z][i]) : 0;
-  int c = cell_key(g, cx, cy, cz);
-  a.cellid[i] = c;
-  atomicAdd(&a.cfill[c], 1);
+  const int i = vl.b * blockDim.x + threadIdx.x;
+  int c = -1;
+  if (i < a.n) {
+    const Grid g = *gp;
+    int cx = cell_coord(g, 0, a.f[F_X][i]);
+    int cy = g.ndim >= 2 ? cell_coord(g, 1, a.f[F_Y][i]) : 0;
+    int cz = g.ndim >= 3 ? cell_coord(g, 2, a.f[F_Z][i]) : 0;
+    c = cell_key(g, cx, cy, cz);
+    a.cellid[i] = c;
+  }
+  run_atomic_add(a.cfill, c);
 }
 __global__ void k_cellid(KArgs a, const Grid* __restrict__ gp) {
   b_cellid(a, gp, VL{(int)blockIdx.x, (int)gridDim.x});
@@ -335,11 +356,10 @@ __global__ void k_scan_add(int* out, const int* bsum, const Grid* __restrict__ g
 }
 
 __device__ __forceinline__ void b_scatter(KArgs a, VL vl) {
-  int i = vl.b * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
-  int c = a.cellid[i];
-  int pos = a.cstart[c] + atomicAdd(&a.cfill[c], 1);
-  a.sidx[pos] = i;
+  const int i = vl.b * blockDim.x + threadIdx.x;
+  const int c = i < a.n ? a.cellid[i] : -1;
+  const int r = run_atomic_add(a.cfill, c);
+  if (c >= 0) a.sidx[a.cstart[c] + r] = i;
 }
 __global__ void k_scatter(KArgs a) {
   b_scatter(a, VL{(int)blockIdx.x, (int)gridDim.x});
@@ -363,6 +383,41 @@ __device__ __forceinline__ void b_cellsort(KArgs a, const Grid* __restrict__ gp,
 }
 __global__ void k_cellsort(KArgs a, const Grid* __restrict__ gp) {
   b_cellsort(a, gp, VL{(int)blockIdx.x, (int)gridDim.x});
+}
+// the same order, with the whole wave sorting each cell of its 64 cells in
+// turn: a cell of at most 64 particles is ranked in registers (each slot
+// index counts the smaller ones of its cell), a larger one is sorted by the
+// first lane as above
+__global__ __launch_bounds__(BS) void k_cellsort_wave(KArgs a, const Grid* __restrict__ gp) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  int b = 0, m = 0;
+  if (c < gp->ncell) {
+    b = a.cstart[c];
+    m = a.cstart[c + 1] - b;
+  }
+  unsigned long long todo = __ballot(m > 1);
+  while (todo) {
+    const int src = __ffsll((long long)todo) - 1;
+    todo &= todo - 1ull;
+    const int cb = __shfl(b, src), cm = __shfl(m, src);
+    if (cm <= 64) {
+      const int v = lane < cm ? a.sidx[cb + lane] : INT_MAX;
+      int r = 0;
+      for (int j = 0; j < cm; j++) r += __shfl(v, j) < v;
+      if (lane < cm) a.sidx[cb + r] = v;
+    } else if (lane == 0) {
+      for (int k = cb + 1; k < cb + cm; k++) {
+        int v = a.sidx[k];
+        int j = k - 1;
+        while (j >= cb && a.sidx[j] > v) {
+          a.sidx[j + 1] = a.sidx[j];
+          j--;
+        }
+        a.sidx[j + 1] = v;
+      }
+    }
+  }
 }
 
 // ===========================================================================
@@ -916,8 +971,24 @@ __device__ __forceinline__ void b_red_final(const double* __restrict__ red, int 
   for (int q = 0; q < spec.nq; q++) {
     const int kind = spec.op[q] & 15, type = spec.op[q] >> 4;
     const double neutral = kind == RED_SUM ? 0.0 : kind == RED_MAX ? -DBL_MAX : DBL_MAX;
-    double x = neutral;
-    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) x = red_combine(kind, x, red[(size_t)q * rcap + b]);
+    // eight independent partial folds per thread keep eight loads in flight
+    // (every fold here is a min, a max or a sum of integers: exact in any order)
+    const double* __restrict__ rq = red + (size_t)q * rcap;
+    double xs[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) xs[u] = neutral;
+    int b = threadIdx.x;
+    for (; b + 7 * BS < nblocks; b += 8 * BS) {
+      double v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = rq[b + u * BS];
+#pragma unroll
+      for (int u = 0; u < 8; u++) xs[u] = red_combine(kind, xs[u], v[u]);
+    }
+    for (; b < nblocks; b += BS) xs[0] = red_combine(kind, xs[0], rq[b]);
+    double x = xs[0];
+#pragma unroll
+    for (int u = 1; u < 8; u++) x = red_combine(kind, x, xs[u]);
     sm[threadIdx.x] = x;
     __syncthreads();
     for (int s = BS / 2; s > 0; s >>= 1) {
@@ -2672,7 +2743,7 @@ static int do_linklist(ndsp_ctx* c, int in_step) {
     hipLaunchKernelGGL(k_fill_cells, dim3(std::min(cap_blocks, 4096u)), dim3(BS), 0, c->st, a.cfill,
                        (const Grid*)c->gdev);
     hipLaunchKernelGGL(k_scatter, dim3(nblk(n)), dim3(BS), 0, c->st, a);
-    hipLaunchKernelGGL(k_cellsort, dim3(cap_blocks), dim3(BS), 0, c->st, a, (const Grid*)c->gdev);
+    hipLaunchKernelGGL(k_cellsort_wave, dim3(cap_blocks), dim3(BS), 0, c->st, a, (const Grid*)c->gdev);
   }
   {
     TScope t(c, "pack_density");
