@@ -187,6 +187,12 @@ int ndsp_kernel_times(ndsp_ctx *ctx, double *ms, int32_t cap, int32_t *n,
  * state is restored afterwards, so results do not change. */
 int ndsp_autotune(ndsp_ctx *ctx, int32_t reps, int32_t *choice);
 
+/* Sets the tile shapes the autotuner chooses among (no counterpart in the
+ * reference): force and h-rho workgroup sizes (64 or 256) and cells per
+ * search radius (1 or 2), e.g. a choice kept from an earlier autotune. */
+int ndsp_set_tile_shapes(ndsp_ctx *ctx, int32_t force_wg, int32_t density_wg,
+                         int32_t cells_per_radius);
+
 /* Test support (no counterpart in the reference): copies nbytes between two
  * fresh device buffers with the library's plain copy kernel, a reference of
  * known traffic for calibrating hardware memory counters. */

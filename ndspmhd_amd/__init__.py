@@ -38,7 +38,7 @@ SYMBOLS = ("ndsp_create", "ndsp_destroy", "ndsp_upload", "ndsp_download",
            "ndsp_mlp_config", "ndsp_mlp_set_params", "ndsp_mlp_get_params",
            "ndsp_mlp_forward", "ndsp_mlp_grad", "ndsp_mlp_train",
            "ndsp_batch_create", "ndsp_batch_destroy", "ndsp_batch_step", "ndsp_batch_autotune", "ndsp_autotune",
-           "ndsp_set_reorder", "ndsp_set_ireal",
+           "ndsp_set_reorder", "ndsp_set_ireal", "ndsp_set_tile_shapes",
            "ndsp_copy_probe",
            "ndsp_last_error")
 MLP_NPARAM = 1313
@@ -89,6 +89,7 @@ def load_library(path: str | None = None):
     L.ndsp_set_graph.argtypes = [vp, ctypes.c_int32]
     L.ndsp_set_reorder.argtypes = [vp, ctypes.c_int32]
     L.ndsp_set_ireal.argtypes = [vp, ip]
+    L.ndsp_set_tile_shapes.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
     L.ndsp_autotune.argtypes = [vp, ctypes.c_int32, ip]
     L.ndsp_copy_probe.argtypes = [vp, ctypes.c_int64]
     L.ndsp_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.POINTER(vp)]
@@ -271,6 +272,11 @@ class Simulation:
         ch = np.zeros(3, dtype=np.int32)
         self._check(load_library().ndsp_autotune(self.h, int(reps), _ip(ch)))
         return {"force_workgroup": int(ch[0]), "density_workgroup": int(ch[1]), "cells_per_radius": int(ch[2])}
+
+    def set_tile_shapes(self, force_workgroup: int, density_workgroup: int, cells_per_radius: int):
+        """tile shapes as autotune returns them (ndsp_set_tile_shapes)"""
+        self._check(load_library().ndsp_set_tile_shapes(self.h, int(force_workgroup), int(density_workgroup),
+                                                         int(cells_per_radius)))
 
     def copy_probe(self, nbytes: int):
         """plain device copy of nbytes (counter calibration, ndsp_copy_probe)"""

@@ -3823,6 +3823,19 @@ static int ensure_key_space(ndsp_ctx* c) {
   return size_key_space(c, hs, 1);
 }
 
+int ndsp_set_tile_shapes(ndsp_ctx* c, int32_t force_wg, int32_t density_wg, int32_t cells_per_radius) {
+  if (!c) return set_err(NDSP_ERR_BAD_ARG, "null context");
+  if ((force_wg != 64 && force_wg != 256) || (density_wg != 64 && density_wg != 256) ||
+      (cells_per_radius != 1 && cells_per_radius != 2))
+    return set_err(NDSP_ERR_BAD_ARG, "tile shapes: workgroups 64 or 256, cells per radius 1 or 2");
+  c->tb_force = force_wg;
+  c->tb_nr = density_wg;
+  c->p.cell_sub = cells_per_radius;
+  c->cap_bits = 0;  // the key space follows the cell size at the next binning
+  c->graph_valid = false;  // recorded steps hold the old launch shapes
+  return 0;
+}
+
 int ndsp_autotune(ndsp_ctx* c, int32_t reps, int32_t* choice) {
   if (!c) return set_err(NDSP_ERR_BAD_ARG, "null context");
   if (reps < 1) reps = 1;
@@ -3868,6 +3881,7 @@ int ndsp_autotune(ndsp_ctx* c, int32_t reps, int32_t* choice) {
   c->tb_nr = bn;
   c->p.cell_sub = bc;
   c->cap_bits = 0;  // the key space follows the chosen cell size at the next binning
+  c->graph_valid = false;
   const int re = restore_state(c, S);
   free_state(S);
   c->timing = timing;

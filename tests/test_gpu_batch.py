@@ -93,6 +93,34 @@ def test_autotune_keeps_state_and_parity():
         assert err <= (1e-9 if f != "x" else 1e-11), (f, err)
 
 
+@pytest.mark.parametrize("shapes", [(64, 64, 1), (256, 256, 2)])
+def test_set_tile_shapes_parity_with_graphs(shapes):
+    """given tile shapes (ndsp_set_tile_shapes) change launch shapes only:
+    graph-replayed steps after the change still match the oracle"""
+    import ndspmhd_amd as nd
+    import oracle
+    from ndspmhd_amd.setups import periodic_mhd_box
+    o, st = periodic_mhd_box(10, seed=3, vel_seed=4)
+    sim = nd.Simulation(o, st, precision=1)
+    orc = oracle.Oracle(o, st)
+    sim.set_graph(True)
+    sim.derivs()
+    orc.derivs()
+    for s in range(4):
+        if s == 2:
+            print("tile shapes", shapes, flush=True)
+            sim.set_tile_shapes(*shapes)
+        sim.set_dt(orc.scalar("dt"), orc.scalar("time"))
+        orc.step()
+        sim.step()
+    for f in ("x", "vel", "Bevol", "en", "hh", "rho", "force"):
+        a, b = sim.get(f), orc.get(f)
+        err = np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+        assert err <= (1e-9 if f != "x" else 1e-11), (f, err)
+    with pytest.raises(Exception):
+        sim.set_tile_shapes(128, 64, 1)
+
+
 def test_batch_autotune_keeps_state_and_parity():
     """the batch tuner restores every member bit for bit (layout included);
     the tuned batch still matches the oracle"""
