@@ -385,7 +385,7 @@ __global__ void k_cellsort(KArgs a, const Grid* __restrict__ gp) {
   b_cellsort(a, gp, VL{(int)blockIdx.x, (int)gridDim.x});
 }
 // the same order, with the whole wave sorting each cell of its 64 cells in
-// turn: a cell of at most 64 particles is ranked in registers (each slot
+// turn: a cell of at most 256 particles is ranked in registers (each slot
 // index counts the smaller ones of its cell), a larger one is sorted by the
 // first lane as above
 __global__ __launch_bounds__(BS) void k_cellsort_wave(KArgs a, const Grid* __restrict__ gp) {
@@ -401,11 +401,28 @@ __global__ __launch_bounds__(BS) void k_cellsort_wave(KArgs a, const Grid* __res
     const int src = __ffsll((long long)todo) - 1;
     todo &= todo - 1ull;
     const int cb = __shfl(b, src), cm = __shfl(m, src);
-    if (cm <= 64) {
-      const int v = lane < cm ? a.sidx[cb + lane] : INT_MAX;
-      int r = 0;
-      for (int j = 0; j < cm; j++) r += __shfl(v, j) < v;
-      if (lane < cm) a.sidx[cb + r] = v;
+    if (cm <= 256) {
+      // up to four entries per lane (positions lane + 64 q)
+      const int v0 = lane < cm ? a.sidx[cb + lane] : INT_MAX;
+      const int v1 = lane + 64 < cm ? a.sidx[cb + lane + 64] : INT_MAX;
+      const int v2 = lane + 128 < cm ? a.sidx[cb + lane + 128] : INT_MAX;
+      const int v3 = lane + 192 < cm ? a.sidx[cb + lane + 192] : INT_MAX;
+      int r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+      for (int q = 0; q * 64 < cm; q++) {
+        const int vq = q == 0 ? v0 : q == 1 ? v1 : q == 2 ? v2 : v3;
+        const int nj = min(64, cm - q * 64);
+        for (int j = 0; j < nj; j++) {
+          const int u = __shfl(vq, j);
+          r0 += u < v0;
+          r1 += u < v1;
+          r2 += u < v2;
+          r3 += u < v3;
+        }
+      }
+      if (lane < cm) a.sidx[cb + r0] = v0;
+      if (lane + 64 < cm) a.sidx[cb + r1] = v1;
+      if (lane + 128 < cm) a.sidx[cb + r2] = v2;
+      if (lane + 192 < cm) a.sidx[cb + r3] = v3;
     } else if (lane == 0) {
       for (int k = cb + 1; k < cb + cm; k++) {
         int v = a.sidx[k];
@@ -2589,6 +2606,7 @@ struct ndsp_ctx {
   bool have_rates = false;
   int nsteps = 0;
   int reorder_every = 10;
+  bool reorder_next = false;  // reorder at the next step (after a change of tile shapes)
   double dt = 0.0, time = 0.0;
   MlpState mlp;
   bool mlp_alloced = false;
@@ -3367,8 +3385,9 @@ int ndsp_step(ndsp_ctx* c, ndsp_stats* stats, ndsp_dt* dtout) {
     // (primitive2conservative, src/conservative2primitive.f90:799)
     if (int e = ndsp_derivs(c, nullptr, nullptr)) return e;
   }
-  if (c->grid_valid && c->reorder_every > 0 && c->nsteps % c->reorder_every == 0) {
+  if (c->grid_valid && c->reorder_every > 0 && (c->nsteps % c->reorder_every == 0 || c->reorder_next)) {
     if (int e = reorder(c)) return e;
+    c->reorder_next = false;
   }
   if (graph_usable(c) && c->cap_bits > 0) {
     // the step as two recorded graphs; the h-rho iteration is completed on
@@ -3708,7 +3727,8 @@ int ndsp_batch_step(ndsp_batch* b, ndsp_stats* stats, ndsp_dt* dt) {
     if (!c->have_rates)
       if (int e = ndsp_derivs(c, nullptr, nullptr)) return e;
     if (int e = ensure_key_space(c)) return e;
-    due[s] = c->grid_valid && c->reorder_every > 0 && c->nsteps % c->reorder_every == 0;
+    due[s] = c->grid_valid && c->reorder_every > 0 && (c->nsteps % c->reorder_every == 0 || c->reorder_next);
+    if (due[s]) c->reorder_next = false;
     if (b->caps.size() != (size_t)ns || b->caps[s] != c->cap_bits) b->valid = false;
   }
   for (int s = 0; s < ns; s++) HIPCHK(hipStreamSynchronize(b->sims[s]->st));
@@ -3833,6 +3853,7 @@ int ndsp_set_tile_shapes(ndsp_ctx* c, int32_t force_wg, int32_t density_wg, int3
   c->p.cell_sub = cells_per_radius;
   c->cap_bits = 0;  // the key space follows the cell size at the next binning
   c->graph_valid = false;  // recorded steps hold the old launch shapes
+  c->reorder_next = true;  // the slot order follows the new cells
   return 0;
 }
 
@@ -3882,6 +3903,7 @@ int ndsp_autotune(ndsp_ctx* c, int32_t reps, int32_t* choice) {
   c->p.cell_sub = bc;
   c->cap_bits = 0;  // the key space follows the chosen cell size at the next binning
   c->graph_valid = false;
+  c->reorder_next = true;  // the restored slot order predates the chosen cells
   const int re = restore_state(c, S);
   free_state(S);
   c->timing = timing;
@@ -3944,6 +3966,7 @@ int ndsp_batch_autotune(ndsp_batch* b, int32_t reps, int32_t* choice) {
   for (int s = 0; s < ns; s++) {
     b->sims[s]->p.cell_sub = bc;
     b->sims[s]->cap_bits = 0;  // sized again at the next step
+    b->sims[s]->reorder_next = true;
   }
   b->valid = false;
   const int re = restore_all();
