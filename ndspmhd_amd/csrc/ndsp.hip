@@ -2771,13 +2771,16 @@ static int do_linklist(ndsp_ctx* c, int in_step) {
       hipLaunchKernelGGL(k_pack_density<float>, dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p);
   }
   if (c->p.use_list) {
-    TScope t(c, "nblist");
     hipLaunchKernelGGL(k_fill_i, dim3(std::min(nblk(n), 4096u)), dim3(BS), 0, c->st, a.nbx, n, 0);
-    dispatch_ndim(c->p.ndim, [&](auto nd) {
-      constexpr int ND = decltype(nd)::value;
-      hipLaunchKernelGGL((k_nblist<ND>), dim3(nblk(n, 64)), dim3(64), 0, c->st, a, c->p, (const Grid*)c->gdev);
-      return 0;
-    });
+    {
+      TScope t(c, "nblist");
+      dispatch_ndim(c->p.ndim, [&](auto nd) {
+        constexpr int ND = decltype(nd)::value;
+        hipLaunchKernelGGL((k_nblist<ND>), dim3(nblk(n, 64)), dim3(64), 0, c->st, a, c->p, (const Grid*)c->gdev);
+        return 0;
+      });
+    }
+    TScope t(c, "nblist_sort");
     hipLaunchKernelGGL(k_nblist_sort, dim3(nblk(n)), dim3(BS), 0, c->st, a);
   }
   HIPCHK(hipGetLastError());
