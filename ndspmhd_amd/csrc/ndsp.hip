@@ -573,12 +573,14 @@ __device__ __forceinline__ unsigned inverse_code(unsigned code) {
 // calling body(kk, r, xj_img, flip, code) with r = load(kk) (a struct holding
 // the FP64 position and 1/h in r.p and whatever the pair sum reads) and the
 // image coordinates of the ghost arithmetic (so the exact FP64 tests see the
-// values of the cell scan).  Entries come four per load; the record of the
-// next entry is fetched while the current one is processed, so a wavefront's
-// gathers overlap its arithmetic.
-template <int NDIM, typename LOAD, typename F>
+// values of the cell scan).  Entries come four per load; the records of the
+// next PF entries (1 or 2) are fetched while the current one is processed, so
+// a wavefront's gathers overlap its arithmetic (two ahead where the pair
+// arithmetic is light and the records small).
+template <int NDIM, int PF = 1, typename LOAD, typename F>
 __device__ __forceinline__ void for_each_listed_pf(const KArgs& a, const Grid& g, int k, int cnt, LOAD&& load,
                                                    F&& body) {
+  static_assert(PF == 1 || PF == 2, "prefetch distance 1 or 2");
   if (cnt <= 0) return;
   const uint4* L4 = (const uint4*)(a.nbl + (size_t)k * a.maxnb);
   const int ng = (cnt + 3) >> 2;
@@ -590,9 +592,11 @@ __device__ __forceinline__ void for_each_listed_pf(const KArgs& a, const Grid& g
   const uint4 g1 = ng > 1 ? L4[1] : make_uint4(0u, 0u, 0u, 0u);
   unsigned c0 = g0.x, c1 = g0.y, c2 = g0.z, c3 = g0.w, n0 = g1.x, n1 = g1.y, n2 = g1.z, n3 = g1.w;
   auto r = load((int)(c0 >> 7));
+  auto r1 = r;
+  if (PF == 2) r1 = load((int)((cnt > 1 ? c1 : c0) >> 7));
   for (int m = 0; m < cnt; m++) {
     const unsigned e = c0;
-    const unsigned en = m + 1 < cnt ? c1 : e;
+    const unsigned en = m + PF < cnt ? (PF == 2 ? c2 : c1) : e;
     const auto rn = load((int)(en >> 7));
     const int kk = (int)(e >> 7);
     const int code = (int)(e & 63u);
@@ -615,7 +619,12 @@ __device__ __forceinline__ void for_each_listed_pf(const KArgs& a, const Grid& g
       }
     }
     if (ok) body(kk, r, xj, flip, code);
-    r = rn;
+    if (PF == 2) {
+      r = r1;
+      r1 = rn;
+    } else {
+      r = rn;
+    }
     c0 = c1; c1 = c2; c2 = c3; c3 = n0; n0 = n1; n1 = n2; n2 = n3;
     if ((m & 3) == 3) {
       const int gi2 = (m >> 2) + 2;
@@ -1112,7 +1121,8 @@ __device__ __forceinline__ void b_nr_pass(KArgs a, Phys p, const Grid* __restric
   // a workgroup with no particle of this pass skips loading the tables (a
   // later pass holds only the particles still iterating); it still writes
   // its neutral reduction partials, which the final fold reads
-  const bool idle = xcd_tile(vl) * (int)blockDim.x >= nlist;
+  const int tile = xcd_tile_n(vl, (nlist + (int)blockDim.x - 1) / (int)blockDim.x);
+  const bool idle = tile * (int)blockDim.x >= nlist;
   __shared__ PT tw[NDSP_IKERN + 1];
   __shared__ PT tg[NDSP_IKERN + 1];
   __shared__ PT tgg[NDSP_IKERN + 1];
@@ -1126,7 +1136,7 @@ __device__ __forceinline__ void b_nr_pass(KArgs a, Phys p, const Grid* __restric
   }
   __syncthreads();
   const Grid g = *gp;
-  const int tid = xcd_tile(vl) * blockDim.x + threadIdx.x;
+  const int tid = tile * (int)blockDim.x + (int)threadIdx.x;
   const bool active = tid < nlist;
   bool unconverged = false;
   const int k = active ? (list ? list[tid] : tid) : 0;
@@ -1230,7 +1240,7 @@ __device__ __forceinline__ void b_nr_pass(KArgs a, Phys p, const Grid* __restric
           drhodt.add(mj * dvdotr * grkerni);
         }
       };
-      if (uselist) for_each_listed_pf<NDIM>(a, g, k, nlisted, loadj, body);
+      if (uselist) for_each_listed_pf<NDIM, 2>(a, g, k, nlisted, loadj, body);
       else for_each_candidate_rec<NDIM>(a, g, xi, rad, loadj, body);
       npairs += (unsigned long long)ngat;
       nn = symp ? nsym : ngat;
@@ -1328,8 +1338,12 @@ __device__ __forceinline__ void b_nr_pass(KArgs a, Phys p, const Grid* __restric
     const int rk[7] = {RED_MAX, RED_MAX, RED_MAX, RED_MIN, RED_MAX, RED_SUM, RED_SUM};
     block_partials<7>(rv, rk, a.red, a.rcap, vl);
   }
-  // append unconverged particles to the next pass's list (one atomic per wave)
-  if (!persistent) {
+  // unconverged particles form the next pass's list: flagged here and
+  // compacted in sorted order afterwards (k_cmp_*), so that the next pass's
+  // workgroups keep the spatial order, or appended one atomic per wave
+  if (!persistent && a.nrflag) {
+    if (active) a.nrflag[k] = unconverged ? 1 : 0;
+  } else if (!persistent) {
     const unsigned long long m = __ballot(unconverged);
     const int lane = threadIdx.x & 63;
     const int cnt = __popcll(m);
@@ -1410,6 +1424,81 @@ __device__ __forceinline__ void b_pass_ctl(DScal* sc, int n, int reflect, int ma
 }
 __global__ void k_pass_ctl(DScal* sc, int n, int reflect, int maxits) {
   b_pass_ctl(sc, n, reflect, maxits, VL{(int)blockIdx.x, (int)gridDim.x});
+}
+
+// Ordered compaction of the flags a pass left (sorted positions of the
+// unconverged particles) into the next pass's list, in three launches over
+// blocks of CMP_ITEMS*BS flags: per-block counts, their exclusive scan (and
+// the total, the next pass's size), and the scatter, which also clears the
+// flags it reads.
+#define CMP_ITEMS 4
+__global__ __launch_bounds__(BS) void k_cmp_count(const int* __restrict__ flag, int n, int* __restrict__ bsum) {
+  __shared__ int ws[BS / 64];
+  const long long base = (long long)blockIdx.x * BS * CMP_ITEMS;
+  int c = 0;
+  for (int q = 0; q < CMP_ITEMS; q++) {
+    const long long i = base + (long long)q * BS + threadIdx.x;
+    if (i < n) c += flag[i];
+  }
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int w = 0; w < BS / 64; w++) t += ws[w];
+    bsum[blockIdx.x] = t;
+  }
+}
+__global__ __launch_bounds__(BS) void k_cmp_scan(int* __restrict__ bsum, int nb, DScal* sc) {
+  __shared__ int sm[BS];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nb; base += BS) {
+    const int idx = base + threadIdx.x;
+    const int v = idx < nb ? bsum[idx] : 0;
+    sm[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 1; s < BS; s <<= 1) {
+      const int x = (int)threadIdx.x >= s ? sm[threadIdx.x - s] : 0;
+      __syncthreads();
+      sm[threadIdx.x] += x;
+      __syncthreads();
+    }
+    if (idx < nb) bsum[idx] = carry + sm[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == BS - 1) carry += sm[BS - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sc->ncalc_next = carry;
+}
+__global__ __launch_bounds__(BS) void k_cmp_scatter(int* __restrict__ flag, int n, const int* __restrict__ bsum,
+                                                    const DScal* __restrict__ sc, int* __restrict__ list_a,
+                                                    int* __restrict__ list_b) {
+  __shared__ int ws[BS / 64];
+  // the list the pass that set the flags names as its next (b_nr_pass)
+  int* __restrict__ out = (sc->pc_its & 1) ? list_a : list_b;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const long long base = (long long)blockIdx.x * BS * CMP_ITEMS;
+  int run = bsum[blockIdx.x];
+  for (int q = 0; q < CMP_ITEMS; q++) {
+    const long long i = base + (long long)q * BS + threadIdx.x;
+    const int f = i < n ? flag[i] : 0;
+    const unsigned long long m = __ballot(f != 0);
+    if (lane == 0) ws[w] = __popcll(m);
+    __syncthreads();
+    int off = run, tot = 0;
+    for (int u = 0; u < BS / 64; u++) {
+      if (u < w) off += ws[u];
+      tot += ws[u];
+    }
+    if (f) {
+      out[off + __popcll(m & ((1ull << lane) - 1ull))] = (int)i;
+      flag[i] = 0;
+    }
+    run += tot;
+    __syncthreads();
+  }
 }
 
 
@@ -2596,6 +2685,7 @@ struct ndsp_ctx {
   double* scratch_d = nullptr;  // REORDER_W fields of n doubles for reorders
   int* scratch_i = nullptr;
   int* bsum = nullptr;
+  int* cmp_bsum = nullptr;      // block offsets of the compaction of unconverged flags
   int* list_a = nullptr;        // active lists of the h-rho passes
   int* list_b = nullptr;
   double* part = nullptr;       // reduction partials
@@ -2807,6 +2897,13 @@ static void launch_passes(ndsp_ctx* c, int K) {
                          c->list_a, c->list_b);
       return 0;
     });
+    {
+      const unsigned nb = (unsigned)((n + BS * CMP_ITEMS - 1) / (BS * CMP_ITEMS));
+      hipLaunchKernelGGL(k_cmp_count, dim3(nb), dim3(BS), 0, c->st, (const int*)a.nrflag, n, c->cmp_bsum);
+      hipLaunchKernelGGL(k_cmp_scan, dim3(1), dim3(BS), 0, c->st, c->cmp_bsum, (int)nb, a.sc);
+      hipLaunchKernelGGL(k_cmp_scatter, dim3(nb), dim3(BS), 0, c->st, a.nrflag, n, (const int*)c->cmp_bsum,
+                         (const DScal*)a.sc, c->list_a, c->list_b);
+    }
     hipLaunchKernelGGL(k_red_final, dim3(1), dim3(BS), 0, c->st, (const double*)a.red, a.rcap, (int)nblk(n, c->tb_nr),
                        red_spec({RED_Q(RED_MAX, RED_F64, stressmax), RED_Q(RED_MAX, RED_F64, hmax_nr),
                                  RED_Q(RED_MAX, RED_I32, itsmax), RED_Q(RED_MIN, RED_I32, nmin),
@@ -3064,6 +3161,9 @@ int ndsp_create(const int32_t* iopt, const double* dopt, int64_t npart, int32_t 
   ok &= hipMalloc(&a.sidx, sizeof(int) * n) == hipSuccess;
   ok &= hipMalloc(&c->list_a, sizeof(int) * n) == hipSuccess;
   ok &= hipMalloc(&c->list_b, sizeof(int) * n) == hipSuccess;
+  ok &= hipMalloc(&a.nrflag, sizeof(int) * n) == hipSuccess;
+  ok &= hipMalloc(&c->cmp_bsum, sizeof(int) * ((size_t)n / (BS * CMP_ITEMS) + 2)) == hipSuccess;
+  if (ok) ok &= hipMemset(a.nrflag, 0, sizeof(int) * n) == hipSuccess;
   ok &= hipMalloc(&a.P4, sizeof(double4) * n) == hipSuccess;
   ok &= hipMalloc(&a.dxb, sizeof(double) * n) == hipSuccess;
   const size_t v4 = sizeof(double4) * n, v1 = sizeof(double) * n;
@@ -3128,7 +3228,7 @@ int ndsp_destroy(ndsp_ctx* c) {
   for (int fi = 0; fi < F_NFIELDS; fi++) if (a.f[fi]) hipFree(a.f[fi]);
   void* ptrs[] = {c->scratch_d, c->scratch_i, a.numneigh, a.nits, a.uid, a.itype, a.ireal, a.slot_of, a.cellid, a.sidx, a.P4, a.dxb,
                   a.D4, a.De, a.G0, a.G1, a.G2, a.G3, a.G4, a.G5, a.GF, a.sc, (void*)a.tabf, (void*)a.tabd,
-                  c->part, c->cons_out, a.cstart, a.cfill, c->bsum, c->list_a, c->list_b,
+                  c->part, c->cons_out, a.cstart, a.cfill, c->bsum, c->list_a, c->list_b, a.nrflag, c->cmp_bsum,
                   a.nbl, a.nbc, a.nbx, a.hcap, a.red, c->gdev};
   for (void* q : ptrs) if (q) hipFree(q);
   if (c->graph_a) hipGraphExecDestroy(c->graph_a);
@@ -3501,6 +3601,7 @@ static int batch_build(ndsp_batch* b) {
       MArgs& M = ma[q];
       memset(&M, 0, sizeof(M));
       M.a = c->a;
+      M.a.nrflag = nullptr;  // batched passes append their unconverged particles
       M.p = c->p;
       M.gdev = c->gdev;
       M.box = c->g;

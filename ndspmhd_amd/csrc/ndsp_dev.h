@@ -226,6 +226,18 @@ __device__ __forceinline__ int xcd_tile(VL vl) {
   return start + q;
 }
 
+// as xcd_tile, for the first nt tiles only (nt <= vl.nb, a pass over part of
+// the particles): every XCD takes a contiguous share of them, where xcd_tile
+// would hand them all to the first XCDs; other workgroups get tiles >= nt
+__device__ __forceinline__ int xcd_tile_n(VL vl, int nt) {
+  const int b = vl.b;
+  const int x = b % NDSP_NXCD, q = b / NDSP_NXCD;
+  if (q >= (nt - x + NDSP_NXCD - 1) / NDSP_NXCD) return nt + b;
+  int start = 0;
+  for (int y = 0; y < x; y++) start += (nt - y + NDSP_NXCD - 1) / NDSP_NXCD;
+  return start + q;
+}
+
 // atomics on non-negative doubles through their IEEE bit patterns
 __device__ __forceinline__ void atomic_max_pos(double* a, double v) {
   atomicMax((unsigned long long*)a, (unsigned long long)__double_as_longlong(v));

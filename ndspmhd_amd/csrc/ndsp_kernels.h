@@ -104,6 +104,7 @@ struct KArgs {
   int* cellid;        // per slot
   int* cstart;        // ncell + 1
   int* cfill;         // ncell
+  int* nrflag;        // unconverged after a pass, by sorted position (compacted in order; null: appended)
   int* sidx;          // sorted position -> slot
   // packed neighbour views in sorted order
   double4* P4;        // x, y, z, 1/h
