@@ -1778,7 +1778,7 @@ __device__ __forceinline__ void b_force(KArgs a, Phys p, const Grid* __restrict_
       }
     };
     if (!err) {
-      if (uselist) for_each_listed_pf<NDIM>(a, g, k, a.nbc[k] + a.nbx[k], loadj, body);
+      if (uselist) for_each_listed_pf<NDIM, 2>(a, g, k, a.nbc[k] + a.nbx[k], loadj, body);
       else for_each_candidate_rec<NDIM>(a, g, xi, rad, loadj, body);
     }
     st_stream(&a.f[F_FX][i], fx.value());
@@ -1988,34 +1988,47 @@ __device__ __forceinline__ void b_predict(KArgs a, Phys p, VL vl) {
   static const int xs[3] = {F_X, F_Y, F_Z}, vs[3] = {F_VX, F_VY, F_VZ}, bs[3] = {F_BX, F_BY, F_BZ};
   static const int as[3] = {F_A0, F_A1, F_A2}, fs[3] = {F_FX, F_FY, F_FZ}, dbs[3] = {F_DBX, F_DBY, F_DBZ};
   static const int das[3] = {F_DA0, F_DA1, F_DA2};
+  // every input into registers first (the field pointers may alias as far as
+  // the compiler knows, so reading back the copies would reload them)
+  double x[3], v[3], b[3], al[3], fo[3], db[3], da[3];
   for (int c = 0; c < 3; c++) {
-    f[F_XIN + c][i] = f[xs[c]][i];
-    f[F_VXIN + c][i] = f[vs[c]][i];
-    f[F_BXIN + c][i] = f[bs[c]][i];
-    f[F_A0IN + c][i] = f[as[c]][i];
-    f[F_FXIN + c][i] = f[fs[c]][i];
-    f[F_DBXIN + c][i] = f[dbs[c]][i];
-    f[F_DA0IN + c][i] = f[das[c]][i];
+    x[c] = f[xs[c]][i];
+    v[c] = f[vs[c]][i];
+    b[c] = f[bs[c]][i];
+    al[c] = f[as[c]][i];
+    fo[c] = f[fs[c]][i];
+    db[c] = f[dbs[c]][i];
+    da[c] = f[das[c]][i];
   }
-  f[F_HIN][i] = f[F_H][i];
-  f[F_ENIN][i] = f[F_EN][i];
-  f[F_PSIIN][i] = f[F_PSI][i];
-  f[F_EPSIN][i] = f[F_EPS][i];
-  f[F_DHDTIN][i] = f[F_DHDT][i];
-  f[F_DENDTIN][i] = f[F_DENDT][i];
-  f[F_DPSIDTIN][i] = f[F_DPSIDT][i];
-  f[F_DEPSIN][i] = f[F_DEPS][i];
-  for (int d = 0; d < p.ndim; d++)
-    f[xs[d]][i] = f[F_XIN + d][i] + dt * f[F_VXIN + d][i] + 0.5 * dt * dt * f[F_FXIN + d][i];
-  for (int c = 0; c < 3; c++) f[vs[c]][i] = (f[F_VXIN + c][i] + dt * f[F_FXIN + c][i]) / 1.0;
+  const double h = f[F_H][i], en = f[F_EN][i], psi = f[F_PSI][i], eps = f[F_EPS][i];
+  const double dhdt = f[F_DHDT][i], dendt = f[F_DENDT][i], dpsidt = f[F_DPSIDT][i], deps = f[F_DEPS][i];
+  for (int c = 0; c < 3; c++) {
+    f[F_XIN + c][i] = x[c];
+    f[F_VXIN + c][i] = v[c];
+    f[F_BXIN + c][i] = b[c];
+    f[F_A0IN + c][i] = al[c];
+    f[F_FXIN + c][i] = fo[c];
+    f[F_DBXIN + c][i] = db[c];
+    f[F_DA0IN + c][i] = da[c];
+  }
+  f[F_HIN][i] = h;
+  f[F_ENIN][i] = en;
+  f[F_PSIIN][i] = psi;
+  f[F_EPSIN][i] = eps;
+  f[F_DHDTIN][i] = dhdt;
+  f[F_DENDTIN][i] = dendt;
+  f[F_DPSIDTIN][i] = dpsidt;
+  f[F_DEPSIN][i] = deps;
+  for (int d = 0; d < p.ndim; d++) f[xs[d]][i] = x[d] + dt * v[d] + 0.5 * dt * dt * fo[d];
+  for (int c = 0; c < 3; c++) f[vs[c]][i] = (v[c] + dt * fo[c]) / 1.0;
   if (p.imhd != 0)
-    for (int c = 0; c < 3; c++) f[bs[c]][i] = f[F_BXIN + c][i] + dt * f[F_DBXIN + c][i];
-  f[F_H][i] = f[F_HIN][i] + dt * f[F_DHDTIN][i];
-  if (p.iener != 0) f[F_EN][i] = f[F_ENIN][i] + dt * f[F_DENDTIN][i];
+    for (int c = 0; c < 3; c++) f[bs[c]][i] = b[c] + dt * db[c];
+  f[F_H][i] = h + dt * dhdt;
+  if (p.iener != 0) f[F_EN][i] = en + dt * dendt;
   for (int c = 0; c < 3; c++)
-    if (p.iavlim[c] != 0) f[as[c]][i] = fmin(f[F_A0IN + c][i] + dt * f[F_DA0IN + c][i], 1.0);
-  if (p.idivbzero >= 2) f[F_PSI][i] = f[F_PSIIN][i] + dt * f[F_DPSIDTIN][i];
-  if (p.idust) f[F_EPS][i] = f[F_EPSIN][i] + dt * f[F_DEPSIN][i];
+    if (p.iavlim[c] != 0) f[as[c]][i] = fmin(al[c] + dt * da[c], 1.0);
+  if (p.idivbzero >= 2) f[F_PSI][i] = psi + dt * dpsidt;
+  if (p.idust) f[F_EPS][i] = eps + dt * deps;
 }
 __global__ void k_predict(KArgs a, Phys p) {
   b_predict(a, p, VL{(int)blockIdx.x, (int)gridDim.x});
