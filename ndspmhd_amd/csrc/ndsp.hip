@@ -1778,7 +1778,9 @@ __device__ __forceinline__ void b_force(KArgs a, Phys p, const Grid* __restrict_
       }
     };
     if (!err) {
-      if (uselist) for_each_listed_pf<NDIM, 2>(a, g, k, a.nbc[k] + a.nbx[k], loadj, body);
+      // two records ahead for FP32 records; an FP64 force record is twice the
+      // registers and one ahead keeps more waves resident
+      if (uselist) for_each_listed_pf<NDIM, sizeof(PT) == 4 ? 2 : 1>(a, g, k, a.nbc[k] + a.nbx[k], loadj, body);
       else for_each_candidate_rec<NDIM>(a, g, xi, rad, loadj, body);
     }
     st_stream(&a.f[F_FX][i], fx.value());
