@@ -938,9 +938,32 @@ __device__ __forceinline__ void b_nblist_sort(KArgs a, VL vl) {
     atomicAdd(&a.sc->list_overflow, 1);
     return;
   }
-  // extras occupy E = L[maxnb-nx .. maxnb-1]: insertion sort in place, then
-  // move down to L[c0 ..] (ascending copy, destination not above source)
+  // extras occupy E = L[maxnb-nx .. maxnb-1]; they end at L[c0 ..] ascending
   unsigned* E = a.nbl + (size_t)k * a.maxnb + (a.maxnb - nx);
+  unsigned* D = a.nbl + (size_t)k * a.maxnb + c0;
+  constexpr int SMAX = 32;
+  if (nx <= SMAX) {
+    // up to 32 of them sorted in registers (odd-even transposition, every
+    // index a constant), then stored in place
+    unsigned v[SMAX];
+#pragma unroll
+    for (int m = 0; m < SMAX; m++) v[m] = m < nx ? E[m] : 0xffffffffu;
+#pragma unroll
+    for (int r = 0; r < SMAX; r++) {
+#pragma unroll
+      for (int m = r & 1; m + 1 < SMAX; m += 2) {
+        const unsigned lo = min(v[m], v[m + 1]), hi = max(v[m], v[m + 1]);
+        v[m] = lo;
+        v[m + 1] = hi;
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < SMAX; m++)
+      if (m < nx) D[m] = v[m];
+    return;
+  }
+  // more: insertion sort in place, then move down (ascending copy,
+  // destination not above source)
   for (int m = 1; m < nx; m++) {
     const unsigned v = E[m];
     int q = m - 1;
@@ -950,7 +973,6 @@ __device__ __forceinline__ void b_nblist_sort(KArgs a, VL vl) {
     }
     E[q + 1] = v;
   }
-  unsigned* D = a.nbl + (size_t)k * a.maxnb + c0;
   if (D != E)
     for (int m = 0; m < nx; m++) D[m] = E[m];
 }
