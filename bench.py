@@ -287,7 +287,7 @@ def main():
         SHAPES = tuple(int(v) for v in args.shapes.split(","))
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is "
-              "design-only (see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+              "a design only. Run with --gpus 1.", file=sys.stderr)
         return 2
 
     log("[bench] mixed precision, %d^3, warmup %d, steps %d" % (args.nside, args.warmup, args.steps))

@@ -54,7 +54,7 @@ FC = os.environ.get("NDSP_FC", "/opt/rocm/bin/amdflang")
 
 
 def build_fortran(verbose: bool = False) -> str:
-    """Builds the Fortran host driver ndspmhd_amd/fortran/ndsp_run against
+    """Builds the Fortran host program ndspmhd_amd/fortran/ndsp_run against
     libndsp_hip.so (iso_c_binding, no compiler-specific extensions)."""
     import tempfile
     if not os.path.exists(FC):

@@ -1,5 +1,5 @@
 !------------------------------------------------------------------------------
-! Fortran driver of the MI355X hot path: the time loop of the reference's
+! Fortran host program of the MI355X hot path: the time loop of the reference's
 ! evolve (src/evolve.f90:52-190) around the argument-less derivs/step entry
 ! points of ndsp_host, on one GPU, with the reference's outputs: the .ev
 ! file (one line per step, evwrite) and dump files (write_dump), and

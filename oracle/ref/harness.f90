@@ -1,5 +1,5 @@
 !------------------------------------------------------------------------------
-! Test-only driver for the compiled reference (ndspmhd src/, built unmodified
+! Test-only main program for the compiled reference (ndspmhd src/, built unmodified
 ! from the reference tree by oracle/ref/Makefile).  It is the oracle's
 ! checker: it writes golden vectors for tests/golden, nothing else uses it.
 !
@@ -13,7 +13,7 @@
 ! (src/ratesND_mhd.f90:415).
 !
 ! The reference's quit (src/ndspmhd.f90:369) lives beside its main program,
-! which this driver replaces, so the driver supplies its own quit that stops
+! which this program replaces, so the program supplies its own quit that stops
 ! with a non-zero status.
 !
 ! Input / output: little-endian stream files, layout documented in

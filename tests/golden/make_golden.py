@@ -1,7 +1,7 @@
 """Writes the golden vectors in tests/golden/*.npz from the compiled
 reference (oracle/_ref/ndsp_ref3d, built by `make -C oracle/ref` from the
 unmodified reference sources).  Run in the container that has the
-reference tree; the GPU box only reads the committed .npz files.
+reference tree; runs on a GPU read only the committed .npz files.
 
     python tests/golden/make_golden.py
 

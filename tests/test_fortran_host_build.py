@@ -1,4 +1,4 @@
-"""The Fortran host driver is built against libndsp_hip.so and resolves it
+"""The Fortran host program is built against libndsp_hip.so and resolves it
 through its rpath (no GPU needed)."""
 import os
 import subprocess
@@ -7,7 +7,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "ndspmhd_amd", "fortran", "ndsp_run")
 
 
-def test_driver_links_library():
+def test_host_program_links_library():
     if not os.path.exists(EXE):
         from ndspmhd_amd.build import build_fortran
         build_fortran()

@@ -114,7 +114,7 @@ def test_fortran_host_ev_and_dumps():
             ra, rb = df.read_records(os.path.join(d, "run_%05d.dat" % k)), df.read_records(
                 os.path.join(d, "py_%05d.dat" % k))
             assert [len(r) for r in ra] == [len(r) for r in rb]
-            # header: time accumulates on the host in the Fortran driver
+            # header: time accumulates on the host in the Fortran program
             assert ra[0][8:] == rb[0][8:], k
             assert ra[1:] == rb[1:], ("dump", k, [i for i, (x, y) in enumerate(zip(ra, rb)) if x != y][:5])
         # restart from the last dump: the state is taken up exactly
