@@ -53,3 +53,26 @@ def test_coincident_particles_reported():
     orc = oracle.Oracle(o, st)
     with pytest.raises(Exception):
         orc.derivs()
+
+
+@pytest.mark.parametrize("hscale", [1.8, 2.6])
+def test_crowded_cells_fp64(hscale):
+    """one particle with a large h makes every cell (side radkern*hmax, one
+    per search radius) hold about 100-200 (1.8) or 300-500 (2.6) particles:
+    the in-cell order is then built by the four-per-lane ranking or the
+    single-lane fallback; results as the oracle's"""
+    import ndspmhd_amd as nd
+    import oracle
+    from ndspmhd_amd.setups import periodic_mhd_box
+    o, st = periodic_mhd_box(12, seed=7, vel_seed=8)
+    st = dict(st)
+    st["hh"] = st["hh"].copy()
+    st["hh"][5] *= hscale
+    print("crowded cells, h scale", hscale, flush=True)
+    sim = nd.Simulation(o, st, precision=1)
+    sim.set_tile_shapes(256, 256, 1)
+    orc = oracle.Oracle(o, st)
+    sim.derivs()
+    orc.derivs()
+    _compare(sim, orc, 1e-10)
+    np.testing.assert_array_equal(sim.get("numneigh").astype(np.int32), orc.get("numneigh").astype(np.int32))
