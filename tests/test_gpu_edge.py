@@ -55,12 +55,14 @@ def test_coincident_particles_reported():
         orc.derivs()
 
 
-@pytest.mark.parametrize("hscale", [1.8, 2.6])
+@pytest.mark.parametrize("hscale", [1.8, 2.6, 0.4])
 def test_crowded_cells_fp64(hscale):
     """one particle with a large h makes every cell (side radkern*hmax, one
-    per search radius) hold about 100-200 (1.8) or 300-500 (2.6) particles:
-    the in-cell order is then built by the four-per-lane ranking or the
-    single-lane fallback; results as the oracle's"""
+    per search radius) hold about 200 (1.8) or all 1728 (2.6) particles: the
+    in-cell order is then built by the four-per-lane ranking or the
+    single-lane fallback.  One with a small h (0.4) sits in the lists of
+    about 60 particles that are not in its own, more transposed extras than
+    the in-register sort takes.  Results as the oracle's"""
     import ndspmhd_amd as nd
     import oracle
     from ndspmhd_amd.setups import periodic_mhd_box
