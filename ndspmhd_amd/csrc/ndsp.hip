@@ -734,20 +734,23 @@ __device__ __forceinline__ void b_nblist(KArgs a, Phys p, const Grid* __restrict
   unsigned* L = a.nbl + (size_t)(act ? k : 0) * a.maxnb;
   int cnt = 0;
   bool over = false;
-  // box of the group's particles widened by the largest search radius
-  double gbox_lo[3], gbox_hi[3];
+  // box of the group's particles and the largest search radius
+  // (candidates are kept within that radius of the box itself, which drops
+  // the corners of the widened box: about a fifth of them)
+  double gbox_lo[3], gbox_hi[3], rw2;
   {
     double rw = rad;
     for (int o = 32; o > 0; o >>= 1) rw = fmax(rw, __shfl_xor(rw, o));
     rw *= 1.0 + 1.0e-9;
+    rw2 = rw * rw;
     for (int d = 0; d < 3; d++) {
       double lo_ = act ? xi[d] : DBL_MAX, hi_ = act ? xi[d] : -DBL_MAX;
       for (int o = 32; o > 0; o >>= 1) {
         lo_ = fmin(lo_, __shfl_xor(lo_, o));
         hi_ = fmax(hi_, __shfl_xor(hi_, o));
       }
-      gbox_lo[d] = lo_ - rw;
-      gbox_hi[d] = hi_ + rw;
+      gbox_lo[d] = lo_;
+      gbox_hi[d] = hi_;
     }
   }
   int fill = 0;  // staged candidates awaiting the test (wave-uniform)
@@ -842,7 +845,12 @@ __device__ __forceinline__ void b_nblist(KArgs a, Phys p, const Grid* __restrict
         if (kd) image_coord(g, d, kd, xj[d], 0.0, xj[d]);
       }
       bool keep = lane < T - q0;
-      for (int d = 0; d < NDIM; d++) keep = keep && xj[d] >= gbox_lo[d] && xj[d] <= gbox_hi[d];
+      double gd2 = 0.0;
+      for (int d = 0; d < NDIM; d++) {
+        const double e = fmax(fmax(gbox_lo[d] - xj[d], xj[d] - gbox_hi[d]), 0.0);
+        gd2 += e * e;
+      }
+      keep = keep && gd2 <= rw2;
       const unsigned long long bal = __ballot(keep);
       if (keep) {
         const int pos = fill + __popcll(bal & ((1ull << lane) - 1ull));
