@@ -21,162 +21,9 @@
 ! the options, as write_infile writes an input file, to <output>.in.
 !------------------------------------------------------------------------------
 program ndsp_ref_harness
- use dimen_mhd
- use debug
- use loguns
- use artvi
- use bound
- use eos
- use hterms
- use linklist
- use options
- use part
- use part_in
- use rates
- use derivB
- use timestep
- use setup_params
- use kernels,        only:setkernels,radkern,radkern2
- use mem_allocation, only:alloc
- use dust,           only:init_drag
- use get_neighbour_lists, only:get_neighbour_list_partial
- use infiles,        only:write_infile
- implicit none
- integer, parameter :: maxopt = 32
- integer :: iopt(maxopt)
- real    :: dopt(maxopt)
- integer :: magic,mode,np,ndimin,ierr,ierr1,ierr2,i,j,n,k,nneigh,icode,ip
- integer :: npg,nps
- integer, allocatable :: listneigh(:),pg(:,:),ps(:,:),irin(:)
- real, allocatable :: dustin(:),zeros(:),zeros3(:,:)
- real :: rij2,q2i,q2j,hi1,hi21,hj1,etot,momtot
- integer(8) :: count0,count1,count_rate
- real, dimension(ndim) :: dx
- character(len=256) :: infile,outfile
+ include 'harness_decl.inc'
 
- call get_command_argument(1,infile)
- call get_command_argument(2,outfile)
- iprint = 6
- trace = .false.
- idebug = '      '
- call logun
-
- open(unit=21,file=trim(infile),access='stream',form='unformatted',status='old',iostat=ierr)
- if (ierr /= 0) error stop 'cannot open input'
- read(21) magic,mode,np,ndimin
- if (magic /= 1313100880) error stop 'bad magic'
- if (ndimin /= ndim) error stop 'input ndim does not match this build'
- read(21) iopt
- read(21) dopt
-!
-!--options: defaults first (src/defaults.f90:29), then the input values
-!
- call set_default_options
- ikernel      = iopt(1)
- ikernelalt   = ikernel
- iener        = iopt(2)
- imhd         = iopt(3)
- imagforce    = iopt(4)
- idivbzero    = iopt(5)
- iav          = iopt(6)
- iavlim(1:3)  = iopt(7:9)
- idust        = iopt(10)
- idrag_nature = iopt(11)
- ibound(1:ndim) = iopt(12:11+ndim)
- maxdensits   = iopt(15)
- idustevol    = iopt(16)
- ivisc = 0; iexternal_force = 0; igravity = 0; iresist = 0; iambipolar = 0
- icty = 0; ikernav = 3; ihvar = 2; iprterm = 0; ixsph = 0; iquantum = 0
- ind_timesteps = 0; nsubsteps_divB = 0; iuse_exact_derivs = 0; idiffuse = 0
- ibiascorrection = 0; icompute_d2v = 0; usenumdens = .false.; damp = 0.
-!--nsteps_remap is only read for imhd = 10, 20, -3, but flang evaluates both
-!  operands of the .and. guarding mod(nsteps,nsteps_remap)
-!  (src/conservative2primitive.f90:138-139), so 0 would divide by zero
- nsteps_remap = 1
- hfact        = dopt(1)
- tolh         = dopt(2)
- gamma        = dopt(3)
- polyk        = dopt(4)
- Kdrag        = dopt(5)
- alphamin     = dopt(6)
- alphaumin    = dopt(7)
- alphabmin    = dopt(8)
- beta         = dopt(9)
- avdecayconst = dopt(10)
- psidecayfact = dopt(11)
- C_cour       = dopt(12)
- C_force      = dopt(13)
- xmin(1:ndim) = dopt(14:13+ndim)
- xmax(1:ndim) = dopt(17:16+ndim)
- psep         = dopt(20)
- dt           = dopt(24)
- nsteps = 0
- time = 0.
- onef_dust = (idust==1 .or. idust==3 .or. idust==4)
- dndim = 1./ndim
- rhomin = 0.
- h_min = 0.
- vsig2max = 0.
- pext = 0.
- geom = 'cartesian'
-!--as in src/initialiseND_mhd.f90:166-172
- if (abs(gamma-1.).gt.1.e-3) then
-    avfact = log(4.)/(log((gamma+1.)/(gamma-1.)))
- else
-    avfact = 1.0
- endif
- call setkernels(ikernel,ikernelalt,ndim,ierr1,ierr2)
- if (ierr1 /= 0 .or. ierr2 /= 0) error stop 'kernel setup failed'
- if (idust /= 0) then
-    call init_drag(ierr,gamma)
-    if (ierr /= 0) error stop 'init_drag failed'
- endif
-!
-!--particles
-!
- call alloc(np)
- npart = np
- ntotal = np
- Bconst(:) = dopt(21:23)
- read(21) x(1:ndim,1:np)
- read(21) vel(1:3,1:np)
- read(21) pmass(1:np)
- read(21) hh(1:np)
- read(21) en(1:np)
- read(21) Bevol(1:3,1:np)
- read(21) alpha(1:3,1:np)
- read(21) psi(1:np)
- allocate(dustin(np),zeros(np),zeros3(3,np))
- zeros = 0.
- zeros3 = 0.
- read(21) dustin(1:np)
- if (onef_dust) dustevol(1,1:np) = dustin(1:np)
- read(21) itype(1:np)
-!--partners of fixed particles (0-based, -1 none) -> ireal (1-based, 0 none),
-!  as set_fixedbound (src/set_fixedbound.f90:32-48) would set them
- allocate(irin(np))
- read(21) irin(1:np)
- ireal(1:np) = irin(1:np) + 1
- close(21)
- rho(1:np) = 0.
- uu(1:np) = en(1:np)
- Bfield(:,1:np) = Bevol(:,1:np)
- if (imhd == 0) then
-    Bfield = 0.
-    Bevol = 0.
- endif
- if (onef_dust) then
-    dustfrac(:,1:np) = dustevol(:,1:np)
-    deltav = 0.
- endif
- sqrtg = 1.
- divB = 0.
- curlB = 0.
- gradpsi = 0.
- force = 0.
- dBevoldt = 0.
- drhodt = 0.
- dhdt = 0.
+ include 'harness_setup.inc'
 !
 !--the hot path: one derivative evaluation (wall time printed for the CPU
 !  baseline)

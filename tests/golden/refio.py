@@ -6,12 +6,16 @@ Input file (little-endian stream):
   float64 dopt[32]     (slots: ndspmhd_amd.options.DOPT)
   float64 x(ndim,N) vel(3,N) pmass(N) hh(N) en(N) Bevol(3,N) alpha(3,N) psi(N) dustevol(N)
   int32   itype(N) ireal(N)  (partner of each fixed particle, 0-based; -1 none)
+Mode 3 is the binding harness (oracle/ref/bindtest.f90: the reference with
+its hot-path files replaced by the GPU binding, stepped as evolve does).
 Output file:
   int32  npart, ntotal, itsdensity, 0
   float64 dtcourant, dtforce, dtdrag, dtav, vsig2max, hhmax
   then OUT_FIELDS below (float64, N*width each), numneigh int32(N);
   mode 2 appends: int32 npg, int32 pg(3,npg), int32 nps, int32 ps(3,nps)
-  (i, j_real, image code), 1-based particle indices.
+  (i, j_real, image code), 1-based particle indices;
+  mode 3 appends: int32 nsteps, float64 time, dt, then vel(3,N) en(N)
+  Bevol(3,N) alpha(3,N) after the last step.
 """
 from __future__ import annotations
 
@@ -63,6 +67,12 @@ def read_output(path, ndim, mode=1):
         a = take("<f8", npart * w)
         out[name] = a.reshape(npart, w) if w > 1 else a
     out["numneigh"] = take("<i4", npart)
+    if mode == 3:
+        out["nsteps"] = int(take("<i4", 1)[0])
+        out["time"], out["dt"] = (float(v) for v in take("<f8", 2))
+        for name, w in (("vel", 3), ("en", 1), ("Bevol", 3), ("alpha", 3)):
+            a = take("<f8", npart * w)
+            out[name] = a.reshape(npart, w) if w > 1 else a
     if mode == 2:
         npg = int(take("<i4", 1)[0])
         out["pairs_gather"] = take("<i4", 3 * npg).reshape(npg, 3)
@@ -96,5 +106,26 @@ def run_reference(binary, opts, st, mode=1, timeout=600):
             out["infile"] = f.read()
         with open(fout + ".in2") as f:
             out["infile2"] = f.read()
+        out["log"] = p.stdout
+        return out
+
+
+def run_binding(binary, opts, st, nsteps, precision="fp64", timeout=600):
+    """Run the binding harness (oracle/ref/bindtest.f90) for nsteps steps;
+    returns the final state and the .ev text written by the reference's
+    evwrite."""
+    with tempfile.TemporaryDirectory() as d:
+        fin = os.path.join(d, "in.bin")
+        fout = os.path.join(d, "out.bin")
+        write_input(fin, opts, st, 3)
+        env = dict(os.environ, NDSP_PRECISION=precision)
+        p = subprocess.run([os.path.abspath(binary), fin, fout, str(nsteps)], cwd=d, capture_output=True,
+                           text=True, timeout=timeout, env=env, preexec_fn=_unlimited_stack)
+        if p.returncode != 0:
+            raise RuntimeError("binding harness failed (%d):\n%s\n%s"
+                               % (p.returncode, p.stdout[-4000:], p.stderr[-4000:]))
+        out = read_output(fout, opts.ndim, 3)
+        with open(fout + ".ev") as f:
+            out["ev"] = f.read()
         out["log"] = p.stdout
         return out

@@ -1,0 +1,64 @@
+"""The reference itself on the GPU: oracle/ref/bindtest.f90 is the reference's
+code (module globals, evwrite) with its hot-path files replaced by the
+binding ndspmhd_amd/fortran/ndsp_refbind.f90, stepped as evolve steps
+(src/evolve.f90:53-106) after the reference's primitive2conservative
+(src/conservative2primitive.f90:478-810) has set it up.  Its final state, its dt sequence and the .ev lines
+the reference's own evwrite writes from the arrays the binding fills agree
+with the CPU oracle stepped with the same dt sequence.  The binary is built
+from the reference tree in the build container and travels with the tree."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "oracle", "_ref", "ndsp_bind3d")
+
+
+def _ev_rows(text):
+    return np.array([[float(v) for v in l.split()] for l in text.splitlines()
+                     if l.strip() and not l.lstrip().startswith("#")])
+
+
+@pytest.mark.skipif(not os.path.exists(BIN), reason="binding harness not built (needs the reference tree)")
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("mixed", 1e-4)])
+def test_reference_evolve_with_binding(prec, tol):
+    import oracle
+    from ndspmhd_amd.setups import periodic_mhd_box
+    from refio import run_binding
+    o, st = periodic_mhd_box(10, seed=3, vel_seed=4)
+    rng = np.random.default_rng(5)
+    st["Bevol"] += 0.03 * rng.standard_normal(st["Bevol"].shape)
+    nsteps = 8
+    print("binding harness: mhd box 10^3, %d steps, %s" % (nsteps, prec), flush=True)
+    out = run_binding(BIN, o, st, nsteps, precision=prec)
+    # primitive2conservative iterates the density and then calls derivs,
+    # which iterates again from the converged h: two evaluations
+    orc = oracle.Oracle(o, st)
+    orc.derivs()
+    orc.derivs()
+    cons = [orc.conserved()]
+    for _ in range(nsteps):
+        orc.step()
+        cons.append(orc.conserved())
+    assert out["nsteps"] == nsteps
+    assert out["time"] == pytest.approx(orc.scalar("time"), rel=10 * tol)
+    assert out["dt"] == pytest.approx(orc.scalar("dt"), rel=10 * tol)
+    bad = []
+    for k in ("x", "vel", "hh", "en", "Bevol", "alpha", "rho", "force", "dBevoldt", "dudt"):
+        a, b = out[k], orc.get(k)
+        a = a.reshape(b.shape)
+        err = float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+        print(k, err, flush=True)
+        if err > (tol if k != "x" else tol * 1e-2):
+            bad.append((k, err))
+    assert not bad, bad
+    # the reference's evwrite, reading the module arrays the binding filled:
+    # one line before the loop and one per step (t ekin etherm emag epot etot)
+    rows = _ev_rows(out["ev"])
+    assert rows.shape[0] == nsteps + 1
+    for r, c in zip(rows, cons):
+        for col, k in ((1, "ekin"), (2, "etherm"), (3, "emag"), (5, "etot")):
+            assert r[col] == pytest.approx(c[k], rel=max(tol, 2e-10)), (k, r[col], c[k])
