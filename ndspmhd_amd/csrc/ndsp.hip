@@ -2134,6 +2134,7 @@ __device__ __forceinline__ void b_correct(KArgs a, Phys p, VL vl) {
   const double hdt = 0.5 * dt;
   static const int vs[3] = {F_VX, F_VY, F_VZ}, bs[3] = {F_BX, F_BY, F_BZ}, as[3] = {F_A0, F_A1, F_A2};
   static const int fs[3] = {F_FX, F_FY, F_FZ}, dbs[3] = {F_DBX, F_DBY, F_DBZ}, das[3] = {F_DA0, F_DA1, F_DA2};
+  f[F_HEVAL][i] = f[F_H][i];
   if (p.fixed_on && a.itype[i] == 1) {
     // fixed particles get back their own starting values
     // (src/stepND_leapfrog_mhd.f90:172-186)
@@ -3356,7 +3357,7 @@ static const struct { const char* name; int f0; int w; } kFieldMap[] = {
     {"dendt", F_DENDT, 1}, {"dBevoldt", F_DBX, 3}, {"daldt", F_DA0, 3}, {"dpsidt", F_DPSIDT, 1},
     {"divB", F_DIVB, 1}, {"curlB", F_CBX, 3}, {"gradpsi", F_GPX, 3}, {"dustfrac", F_DUSTFRAC, 1},
     {"rhogas", F_RHOGAS, 1}, {"rhodust", F_RHODUST, 1}, {"ddustevoldt", F_DEPS, 1},
-    {"deltav", F_DVX, 3}, {"del2u", F_DEL2U, 1}, {"graddivv", F_GDVX, 3}};
+    {"deltav", F_DVX, 3}, {"del2u", F_DEL2U, 1}, {"graddivv", F_GDVX, 3}, {"hh_eval", F_HEVAL, 1}};
 
 int ndsp_download(ndsp_ctx* c, const char* name, double* out, int32_t* width) {
   if (!c || !name || !out) return set_err(NDSP_ERR_BAD_ARG, "download: null argument");
@@ -3465,7 +3466,8 @@ static int step_part_a(ndsp_ctx* c) {
     TScope t(c, "mlp_seed");
     hipLaunchKernelGGL(k_mlp_features, dim3(nblk(c->n)), dim3(BS), 0, c->st, c->a, c->p, c->dt, nexp_of(c->p),
                        ml.X);
-    if (ml.have_params && ml.steps_seen >= ml.seed_after) mlp_launch_seed(ml, c->n, c->a.f[F_H], c->st);
+    if (ml.have_params && ml.steps_seen >= ml.seed_after)
+      mlp_launch_seed(ml, c->n, c->a.f[F_H], c->p.fixed_on ? c->a.itype : nullptr, c->st);
   }
   if (int e = do_linklist(c, 1)) return e;
   if (int e = do_density(c)) return e;

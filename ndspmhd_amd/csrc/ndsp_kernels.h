@@ -20,6 +20,7 @@ enum Field {
   // which the outputs report, as uu is en at the last evaluation (output only)
   F_BFX, F_BFY, F_BFZ,
   F_MLPCORR,  // last step's log(h_converged / h_predictor), an MLP feature
+  F_HEVAL,    // h of the last step's evaluation (the corrector then advances h)
   F_NPERSIST,  // fields above are carried by a reorder
   // step in-copies (src/stepND_leapfrog_mhd.f90:70-100)
   F_XIN = F_NPERSIST, F_YIN, F_ZIN, F_VXIN, F_VYIN, F_VZIN, F_BXIN, F_BYIN, F_BZIN,

@@ -36,7 +36,7 @@ struct MlpState {
 hipError_t mlp_alloc(MlpState& s, int n);
 void mlp_free(MlpState& s);
 hipError_t mlp_set_params(MlpState& s, const float* host, hipStream_t st);
-void mlp_launch_seed(MlpState& s, int n, double* h, hipStream_t st);
+void mlp_launch_seed(MlpState& s, int n, double* h, const int* itype, hipStream_t st);
 void mlp_launch_forward(MlpState& s, const float* X, int ldx, int B, float* y, hipStream_t st);
 void mlp_launch_grad(MlpState& s, const float* X, int nrows, int B, long long off, long long stride,
                      hipStream_t st);

@@ -80,19 +80,23 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const uint16_t* __restrict_
   y[b] = mlp_fwd(sw, x, z1, a1, z2, a2);
 }
 
-// seed of the h-rho iteration: h = h_pred exp(y / scale), |y/scale| <= 0.1
+// seed of the h-rho iteration: h = h_pred exp(y / scale), |y/scale| <= 0.1,
+// applied to the predictor's FP64 h in place (a zero correction leaves it
+// unchanged); fixed boundary particles (itype 1, when given) keep the h the
+// predictor reset them to (src/stepND_leapfrog_mhd.f90:136)
 __global__ __launch_bounds__(256) void k_mlp_seed(const uint16_t* __restrict__ wbf, const float* __restrict__ X,
-                                                  int n, double* __restrict__ h) {
+                                                  int n, double* __restrict__ h, const int* __restrict__ itype) {
   __shared__ float sw[MLP_NPARAM];
   load_weights(wbf, sw);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (itype && itype[i] == 1) return;
   float x[NIN], z1[NH], a1[NH], z2[NH], a2[NH];
   for (int q = 0; q < NIN; q++) x[q] = X[(size_t)i * MLP_LDX + q];
   float y = mlp_fwd(sw, x, z1, a1, z2, a2) / MLP_TSCALE;
   y = fminf(fmaxf(y, -0.1f), 0.1f);
-  const double hpred = (double)X[(size_t)i * MLP_LDX + MLP_COL_HPRED];
-  if (isfinite(y) && hpred > 0.0) h[i] = hpred * exp((double)y);
+  const double hpred = h[i];
+  if (isfinite(y) && y != 0.0f && hpred > 0.0) h[i] = hpred * exp((double)y);
 }
 
 // gradient of L = sum (y - t)^2 / B over the samples idx(b), b < B; sample b
@@ -233,8 +237,9 @@ hipError_t mlp_set_params(MlpState& s, const float* host, hipStream_t st) {
   return hipGetLastError();
 }
 
-void mlp_launch_seed(MlpState& s, int n, double* h, hipStream_t st) {
-  hipLaunchKernelGGL(k_mlp_seed, dim3(nb(n, 256)), dim3(256), 0, st, (const uint16_t*)s.wbf, (const float*)s.X, n, h);
+void mlp_launch_seed(MlpState& s, int n, double* h, const int* itype, hipStream_t st) {
+  hipLaunchKernelGGL(k_mlp_seed, dim3(nb(n, 256)), dim3(256), 0, st, (const uint16_t*)s.wbf, (const float*)s.X, n, h,
+                     itype);
 }
 
 void mlp_launch_forward(MlpState& s, const float* X, int ldx, int B, float* y, hipStream_t st) {

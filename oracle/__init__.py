@@ -40,6 +40,8 @@ def lib():
         L.orc_set_state.argtypes = [ctypes.c_void_p] + [dp] * 9 + [ip]
         L.orc_derivs.argtypes = [ctypes.c_void_p]
         L.orc_step.argtypes = [ctypes.c_void_p]
+        L.orc_predict.argtypes = [ctypes.c_void_p]
+        L.orc_set.argtypes = [ctypes.c_void_p, ctypes.c_char_p, dp]
         L.orc_get.argtypes = [ctypes.c_void_p, ctypes.c_char_p, dp]
         L.orc_scalar.restype = ctypes.c_double
         L.orc_scalar.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
@@ -112,6 +114,10 @@ class Oracle:
     def step(self):
         self._check(lib().orc_step(self.h), "step")
 
+    def predict(self):
+        """the predictor half of a step alone (no evaluation, no corrector)"""
+        self._check(lib().orc_predict(self.h), "predict")
+
     def get(self, name):
         w = 3 if name in FIELDS3 else (self.opts.ndim if name == "x" else 1)
         out = np.empty(self.n * w)
@@ -119,6 +125,14 @@ class Oracle:
         if r < 0:
             raise KeyError(name)
         return out.reshape(self.n, w) if w > 1 else out
+
+    def set(self, name, values):
+        """overwrite a per-particle field (state fields and the rates a step
+        starts from)"""
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        w = 3 if name in FIELDS3 else (self.opts.ndim if name == "x" else 1)
+        if a.size != self.n * w or lib().orc_set(self.h, name.encode(), _dp(a)) < 0:
+            raise KeyError(name)
 
     def scalar(self, name):
         return lib().orc_scalar(self.h, name.encode())

@@ -1291,10 +1291,11 @@ int orc_derivs(orc_sim *s) {
   return 0;
 }
 
-/* step, stepND_leapfrog_mhd.f90:31-275 (damp = 0) */
-int orc_step(orc_sim *s) {
+/* predictor half of step, stepND_leapfrog_mhd.f90:70-165: starting values
+ * kept, state advanced by dt with the starting rates */
+static void predict(orc_sim *s) {
   const int npart = s->npart, ndim = s->ndim;
-  const double dt = s->dt, hdt = 0.5 * dt;
+  const double dt = s->dt;
   for (int i = 0; i < npart; i++) {
     for (int c = 0; c < 3; c++) {
       s->xin[i * 3 + c] = s->x[i * 3 + c];
@@ -1349,6 +1350,18 @@ int orc_step(orc_sim *s) {
     if (s->idivbzero >= 2) s->psi[i] = s->psiin[i] + dt * s->dpsidtin[i];
     if (s->onef_dust) s->dustevol[i] = s->dustevolin[i] + dt * s->ddustevoldtin[i];
   }
+}
+
+int orc_predict(orc_sim *s) {
+  predict(s);
+  return 0;
+}
+
+/* step, stepND_leapfrog_mhd.f90:31-275 (damp = 0) */
+int orc_step(orc_sim *s) {
+  const int npart = s->npart, ndim = s->ndim;
+  const double dt = s->dt, hdt = 0.5 * dt;
+  predict(s);
   int e = orc_derivs(s);
   if (e) return e;
   /* corrector, :169-218; fixed particles (:172-186) get back their own
@@ -1537,6 +1550,27 @@ int orc_get(const orc_sim *s, const char *name, double *out) {
   for (unsigned k = 0; k < sizeof t / sizeof t[0]; k++) {
     if (strcmp(name, t[k].nm)) continue;
     memcpy(out, t[k].p, sizeof(double) * (size_t)n * t[k].w);
+    return t[k].w;
+  }
+  return -1;
+}
+
+int orc_set(orc_sim *s, const char *name, const double *in) {
+  const int n = s->npart;
+  struct { const char *nm; double *p; int w; } t[] = {
+      {"vel", s->vel, 3}, {"hh", s->hh, 1}, {"en", s->en, 1}, {"Bevol", s->Bevol, 3},
+      {"alpha", s->alpha, 3}, {"psi", s->psi, 1}, {"dustevol", s->dustevol, 1},
+      {"rho", s->rho, 1}, {"drhodt", s->drhodt, 1}, {"dhdt", s->dhdt, 1},
+      {"force", s->force, 3}, {"dendt", s->dendt, 1}, {"dBevoldt", s->dBevoldt, 3},
+      {"daldt", s->daldt, 3}, {"dpsidt", s->dpsidt, 1}, {"ddustevoldt", s->ddustevoldt, 1}};
+  if (!strcmp(name, "x")) {
+    for (int i = 0; i < n; i++)
+      for (int d = 0; d < s->ndim; d++) s->x[i * 3 + d] = in[i * s->ndim + d];
+    return s->ndim;
+  }
+  for (unsigned k = 0; k < sizeof t / sizeof t[0]; k++) {
+    if (strcmp(name, t[k].nm)) continue;
+    memcpy(t[k].p, in, sizeof(double) * (size_t)n * t[k].w);
     return t[k].w;
   }
   return -1;

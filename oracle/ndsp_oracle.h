@@ -41,12 +41,23 @@ int orc_derivs(orc_sim *s);
  * (src/stepND_leapfrog_mhd.f90:31-275).  0 = ok. */
 int orc_step(orc_sim *s);
 
+/* the predictor half of a step alone (stepND_leapfrog_mhd.f90:70-165): the
+ * state advanced by dt with the rates of the last evaluation, no evaluation
+ * and no corrector.  0 = ok. */
+int orc_predict(orc_sim *s);
+
 /* copy a per-particle field of the real particles; width 1, 3 or ndim.
  * Names: x vel pmass hh en Bevol alpha psi dustevol rho gradh gradgradh
  * drhodt dhdt dens pr spsound uu Bfield force dudt dendt dBevoldt daldt dpsidt
  * divB curlB gradpsi dustfrac rhogas rhodust ddustevoldt deltav numneigh(its
  * as double) nits.  Returns the width, or -1 for an unknown name. */
 int orc_get(const orc_sim *s, const char *name, double *out);
+
+/* overwrite a per-particle field of the real particles (the state fields,
+ * rho, drhodt and the rates a step starts from: vel hh en Bevol alpha psi
+ * dustevol rho drhodt dhdt force dendt dBevoldt daldt dpsidt ddustevoldt,
+ * and x).  Returns the width, or -1 for a name it does not take. */
+int orc_set(orc_sim *s, const char *name, const double *in);
 
 /* scalars: dt dtcourant dtforce dtdrag dtav vsig2max vsigmax hhmax
  * itsdensity ntotal stressmax time; NaN for an unknown name */

@@ -113,3 +113,122 @@ def test_trained_seed_saves_iterations():
           flush=True)
     assert extra["plain"] > 0.01
     assert extra["mlp"] <= extra["plain"] / 3.0, extra
+
+
+_STATE = ("x", "vel", "pmass", "hh", "en", "Bevol", "alpha", "psi", "dustevol")
+_RATES_IN = ("rho", "drhodt", "dhdt", "force", "dendt", "dBevoldt", "daldt", "dpsidt")
+_EVAL = ("rho", "gradh", "drhodt", "dhdt", "pr", "spsound", "force", "dudt", "dBevoldt", "daldt",
+         "divB", "curlB")
+# norm-relative L-infinity, as tests/test_gpu_parity.py's TOL_MIXED
+_TOL = {"rho": 2e-6, "gradh": 2e-6, "drhodt": 1e-5, "dhdt": 1e-5, "pr": 2e-6, "spsound": 2e-6,
+        "force": 1e-5, "dudt": 1e-5, "dBevoldt": 1e-5, "daldt": 1e-5, "divB": 1e-5, "curlB": 1e-5}
+
+
+def _trained_params():
+    """a network trained online on a turbulent 24^3 MHD box (12 steps)"""
+    import ndspmhd_amd as nd
+    from ndspmhd_amd.setups import periodic_mhd_box
+    o, st = periodic_mhd_box(24, seed=21, vel_seed=22)
+    tr = nd.Simulation(o, st)
+    tr.mlp_config(enable=True, train=True, lr=2e-3, batch=8192, iters=8, seed_after=4)
+    tr.derivs()
+    for _ in range(12):
+        tr.step()
+    p, k = tr.mlp_get_params()
+    assert k > 0
+    return p
+
+
+def _check_seeded_step(o, st, sim, dt, what):
+    """one step of sim with the seed on, checked against the FP64 oracle: the
+    oracle predicts from the same starting state and rates; at the h the GPU
+    iteration stopped at (hh_eval), every evaluated field meets the mixed
+    tolerances for every particle, and the reference's iteration started
+    there moves no h by more than 2 tolh (it is a converged root)"""
+    import oracle
+    pre = {k: sim.get(k) for k in _STATE}
+    pre["itype"] = st["itype"]
+    if "ireal" in st:
+        pre["ireal"] = st["ireal"]
+    orc = oracle.Oracle(o.replace(maxdensits=0), pre)
+    for k in _RATES_IN:
+        orc.set(k, sim.get(k))
+    orc.set_scalar("dt", dt)
+    orc.predict()
+    sim.set_dt(dt, 0.0)
+    sim.step()
+    heval = sim.get("hh_eval")
+    pred = {k: orc.get(k) for k in _STATE}
+    pred["x"] = pred["x"].reshape(st["x"].shape)
+    pred["itype"] = st["itype"]
+    if "ireal" in st:
+        pred["ireal"] = st["ireal"]
+    pred["hh"] = heval
+    at = oracle.Oracle(o.replace(maxdensits=0), pred)
+    at.derivs()
+    bad = []
+    for k in _EVAL:
+        a, b = sim.get(k), at.get(k)
+        err = float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+        if err > _TOL[k]:
+            bad.append((k, err))
+    it = oracle.Oracle(o, pred)
+    it.derivs()
+    dh = float(np.max(np.abs(it.get("hh") - heval) / heval))
+    print(what, "NR passes", sim.stats["itsdensity"], "ncalc_mean", "%.4f" % sim.stats["ncalc_mean"],
+          "oracle from hh_eval: max |dh|/h %.2e" % dh, "field errors ok" if not bad else bad, flush=True)
+    assert not bad, (what, bad)
+    assert dh <= 2 * o.tolh, (what, dh)
+    return heval
+
+
+@pytest.mark.parametrize("name", ["mhd3d_nr", "mhd3d_tight", "mhd3d_resist", "mhd3d_fixedwall"])
+def test_trained_seed_on_golden_boxes(name):
+    """the golden MHD boxes (h-rho iteration from the set-up's h): after the
+    plain first evaluation, a step of dt = 0 re-runs the iteration from the
+    trained network's seed at the same state; it stops within 2 tolh of the
+    reference's converged h, and its fields meet the mixed tolerances at the
+    h it stopped at.  Fixed wall particles keep their h (the seed skips
+    them).  psi cleaning is switched off: the reference's partial passes
+    zero psi where they touch (src/density_sums.f90:630-635), which would
+    make the comparison at fixed h depend on the path."""
+    import ndspmhd_amd as nd
+    import golden_util as gu
+    p = _trained_params()
+    o, st, ref = gu.load(name)
+    o = o.replace(idivbzero=0)
+    st = dict(st)
+    st["psi"] = np.zeros_like(st["psi"])
+    print("golden", name, "N", st["pmass"].shape[0], "tolh", o.tolh, flush=True)
+    sim = nd.Simulation(o, st)
+    sim.mlp_config(enable=True, train=False, seed_after=0)
+    sim.mlp_set_params(p)
+    sim.derivs()
+    h1 = sim.get("hh")
+    assert np.max(np.abs(h1 - ref["hh"]) / ref["hh"]) <= 2 * o.tolh
+    heval = _check_seeded_step(o, st, sim, 0.0, name)
+    assert np.max(np.abs(heval - ref["hh"]) / ref["hh"]) <= 2 * o.tolh
+    if "ireal" in st:
+        fixed = st["itype"] == 1
+        np.testing.assert_array_equal(heval[fixed], heval[st["ireal"][fixed]])
+
+
+def test_trained_seed_steps_against_oracle():
+    """config 4's seeded iteration on a turbulent box of the benchmark's kind
+    (24^3, v_rms Mach 1): three steps with the trained seed, each checked
+    against the oracle from the same starting state"""
+    import ndspmhd_amd as nd
+    from ndspmhd_amd.setups import periodic_mhd_box
+    p = _trained_params()
+    o, st = periodic_mhd_box(24, seed=31, vel_seed=32)
+    o = o.replace(idivbzero=0)
+    sim = nd.Simulation(o, st)
+    sim.mlp_config(enable=True, train=True, lr=2e-3, batch=8192, iters=4, seed_after=0)
+    sim.mlp_set_params(p)
+    sim.derivs()
+    for s in range(2):
+        sim.step()
+    for s in range(3):
+        dt = sim.dt_info["dt"]
+        assert dt > 0
+        _check_seeded_step(o, st, sim, dt, "step %d" % s)

@@ -72,15 +72,35 @@ def test_iterated_density_fp64(name):
 @pytest.mark.parametrize("name", [n for n in gu.names() if not n.endswith("fixedh")])
 def test_iterated_density_mixed(name):
     """FP32 pair sums: every particle converges to the same tolerance.  A
-    convergence decision can only flip within tolh, so h agrees to 2 tolh
-    everywhere; where the iteration took the same path (h within 1e-5) the
-    fields meet the mixed-precision tolerances."""
+    convergence decision (|h_new - h|/h_in < tolh, src/iterate_density.f90:
+    242-243) can flip only where FP32 sums move that ratio across tolh, so h
+    agrees with the reference's to 2 tolh everywhere.  Every particle's
+    fields then meet the mixed tolerances against the FP64 oracle evaluated
+    at the h the GPU iteration stopped at (sums at a given h do not depend on
+    the path to it; psi as the GPU's partial passes left it).  At tolh 1e-6
+    the flips are rare enough that every particle also meets them against
+    the golden vectors directly."""
+    import oracle
     o, st, ref, sim = _sim(name, 0)
     h = sim.get("hh")
     dh = np.abs(h - ref["hh"]) / ref["hh"]
+    print("max |dh|/h vs golden %.3e, particles off the golden path (>1e-9): %d of %d"
+          % (dh.max(), int(np.sum(dh > 1e-9)), dh.size), flush=True)
     assert dh.max() <= 2 * o.tolh
-    same = dh <= 1e-5
-    assert same.mean() >= 0.95
-    for k in DENS + ["force", "dBevoldt", "dudt"]:
-        err = gu.relerr(sim.get(k)[same], ref[k][same])
-        assert err <= 10 * TOL_MIXED[k], (k, err)
+    at = dict(st)
+    at["hh"] = h.copy()
+    at["psi"] = sim.get("psi")
+    orc = oracle.Oracle(o.replace(maxdensits=0), at)
+    orc.derivs()
+    bad = []
+    for k in DENS + ["gradgradh"] + _rates(o):
+        err = gu.relerr(sim.get(k), orc.get(k))
+        print(k, "vs oracle at the GPU's h", err, flush=True)
+        if err > TOL_MIXED[k]:
+            bad.append((k, err))
+    assert not bad, bad
+    if o.tolh <= 1e-6:
+        for k in DENS + _rates(o):
+            err = gu.relerr(sim.get(k), ref[k])
+            print(k, "vs golden", err, flush=True)
+            assert err <= TOL_MIXED[k], (k, err)
