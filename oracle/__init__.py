@@ -17,7 +17,7 @@ _LIB = os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 FIELDS3 = ("vel", "Bevol", "alpha", "Bfield", "force", "dBevoldt", "daldt",
-           "curlB", "gradpsi", "deltav")
+           "curlB", "gradpsi", "deltav", "graddivv")
 
 
 def build(force: bool = False) -> str:

@@ -62,7 +62,7 @@ struct orc_sim {
   double *Bfield, *alpha, *psi, *gradh, *gradgradh, *force, *drhodt, *dudt;
   double *dendt, *dBevoldt, *daldt, *dpsidt, *gradpsi, *dhdt, *divB, *curlB;
   double *dustfrac, *dustevol, *ddustevoldt, *deltav, *rhogas, *rhodust;
-  double *h1, *del2u, *hhin_nr;
+  double *h1, *del2u, *hhin_nr, *graddivv;
   int *itype, *ireal, *numneigh, *ll, *iamincell, *nits, *redolist, *redoprev;
   int *listneigh;
   /* cells, linkND.f90:45-161 */
@@ -102,7 +102,7 @@ static double **dfield(orc_sim *s, int k, int *w) {
   static const int widths[NDFIELDS] = {3, 3, 1, 1, 1, 1, 1, 1, 1, 1, 3, 3,
                                        3, 1, 1, 1, 3, 1, 1, 1, 3, 3, 1, 3,
                                        1, 1, 3, 1, 1, 1, 3, 1, 1, 1, 1, 1,
-                                       1, 1};
+                                       3, 1};
   double **p[NDFIELDS] = {
       &s->x,      &s->vel,      &s->pmass,    &s->rho,     &s->hh,
       &s->uu,     &s->en,       &s->pr,       &s->spsound, &s->dens,
@@ -111,11 +111,11 @@ static double **dfield(orc_sim *s, int k, int *w) {
       &s->dBevoldt, &s->daldt,  &s->dpsidt,   &s->gradpsi, &s->dhdt,
       &s->divB,   &s->curlB,    &s->dustfrac, &s->dustevol, &s->ddustevoldt,
       &s->deltav, &s->rhogas,   &s->rhodust,  &s->h1,      &s->del2u,
-      &s->hhin_nr, &s->hhin_nr, &s->hhin_nr};
+      &s->hhin_nr, &s->graddivv, &s->hhin_nr};
   *w = widths[k];
   return p[k];
 }
-#define NDUNIQUE 36 /* the last two entries above are padding */
+#define NDUNIQUE 37 /* the last entry above is padding */
 
 static int grow(orc_sim *s, int newcap) {
   if (newcap <= s->cap) return 0;
@@ -880,6 +880,7 @@ static int get_rates(orc_sim *s) {
     }
     s->dudt[i] = 0.; s->dendt[i] = 0.; s->dpsidt[i] = 0.; s->divB[i] = 0.;
     s->del2u[i] = 0.;
+    for (int c = 0; c < 3; c++) s->graddivv[i * 3 + c] = 0.;
     s->h1[i] = 1. / s->hh[i];
     if (s->onef_dust) { s->ddustevoldt[i] = 0.; }
   }
@@ -1164,6 +1165,18 @@ static int get_rates(orc_sim *s) {
           double graduterm = (s->uu[i] - s->uu[j]) / rij;
           s->del2u[i] = s->del2u[i] + pmassj * rho1j * graduterm * grkerni;
           s->del2u[j] = s->del2u[j] - pmassi * rho1i * graduterm * grkernj;
+        }
+        if (s->iavlim[0] == 3) {
+          for (int c = 0; c < 3; c++) {
+            s->graddivv[i * 3 + c] = s->graddivv[i * 3 + c] + pmassj * rho1j / rij * dvdotr * grkerni * dr[c];
+            s->graddivv[j * 3 + c] = s->graddivv[j * 3 + c] - pmassi * rho1i / rij * dvdotr * grkernj * dr[c];
+          }
+        } else {
+          /* the "curl v" sum, :1653-1654 */
+          for (int c = 0; c < 3; c++) {
+            s->graddivv[i * 3 + c] = s->graddivv[i * 3 + c] + pmassj * (dvel[c] - dvdotr) * grkerni;
+            s->graddivv[j * 3 + c] = s->graddivv[j * 3 + c] + pmassi * (-dvel[c] - dvdotr) * grkernj;
+          }
         }
       }
       for (int c = 0; c < 3; c++) {
@@ -1536,7 +1549,7 @@ int orc_get(const orc_sim *s, const char *name, double *out) {
       {"dpsidt", s->dpsidt, 1}, {"divB", s->divB, 1}, {"curlB", s->curlB, 3},
       {"gradpsi", s->gradpsi, 3}, {"dustfrac", s->dustfrac, 1}, {"rhogas", s->rhogas, 1},
       {"rhodust", s->rhodust, 1}, {"ddustevoldt", s->ddustevoldt, 1},
-      {"deltav", s->deltav, 3}, {"del2u", s->del2u, 1}};
+      {"deltav", s->deltav, 3}, {"del2u", s->del2u, 1}, {"graddivv", s->graddivv, 3}};
   if (!strcmp(name, "x")) {
     for (int i = 0; i < n; i++)
       for (int d = 0; d < s->ndim; d++) out[i * s->ndim + d] = s->x[i * 3 + d];

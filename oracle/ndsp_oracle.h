@@ -49,8 +49,8 @@ int orc_predict(orc_sim *s);
 /* copy a per-particle field of the real particles; width 1, 3 or ndim.
  * Names: x vel pmass hh en Bevol alpha psi dustevol rho gradh gradgradh
  * drhodt dhdt dens pr spsound uu Bfield force dudt dendt dBevoldt daldt dpsidt
- * divB curlB gradpsi dustfrac rhogas rhodust ddustevoldt deltav numneigh(its
- * as double) nits.  Returns the width, or -1 for an unknown name. */
+ * divB curlB gradpsi dustfrac rhogas rhodust ddustevoldt deltav graddivv
+ * numneigh(its as double) nits.  Returns the width, or -1 for an unknown name. */
 int orc_get(const orc_sim *s, const char *name, double *out);
 
 /* overwrite a per-particle field of the real particles (the state fields,

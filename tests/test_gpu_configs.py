@@ -28,7 +28,10 @@ def _rell1(a, b):
 
 @pytest.mark.parametrize("prec", [1, 0])
 def test_brio_wu_800_to_t01(prec):
-    """FP64: every field within 1e-9 of the oracle at t = 0.1 (1100+ steps).
+    """FP64: the state within 1e-9 of the oracle at t = 0.1 (774 steps; measured
+    <= 5e-11), the rates within 1e-8 (they difference the state across the
+    shock and the contact, which amplifies the round-off of the different
+    summation orders; measured <= 2e-9).
     Mixed: a convergence decision of the h-rho iteration (tolh 1e-3) can
     flip where FP32 pair sums move |h_new - h|/h_in across tolh, which moves
     that particle's h by up to tolh; the runs then stay within 2e-3 in the
@@ -58,7 +61,7 @@ def test_brio_wu_800_to_t01(prec):
         einf, e1 = _relinf(a, b), _rell1(a, b)
         print(k, "Linf", einf, "L1", e1, flush=True)
         if prec == 1:
-            if einf > 1e-9:
+            if einf > (1e-8 if k in ("force", "dBevoldt", "dudt") else 1e-9):
                 bad.append((k, einf))
         elif k == "x":
             if einf > 1e-4:

@@ -166,9 +166,14 @@ def _check_seeded_step(o, st, sim, dt, what):
     pred["hh"] = heval
     at = oracle.Oracle(o.replace(maxdensits=0), pred)
     at.derivs()
+    # the corrector gives fixed particles back their starting rho
+    # (src/stepND_leapfrog_mhd.f90:172-186): theirs is not the evaluation's
+    live = st["itype"] != 1
     bad = []
     for k in _EVAL:
         a, b = sim.get(k), at.get(k)
+        if k == "rho":
+            a, b = a[live], b[live]
         err = float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
         if err > _TOL[k]:
             bad.append((k, err))

@@ -18,8 +18,14 @@ BIN = os.path.join(ROOT, "oracle", "_ref", "ndsp_bind3d")
 
 
 def _ev_rows(text):
-    return np.array([[float(v) for v in l.split()] for l in text.splitlines()
-                     if l.strip() and not l.lstrip().startswith("#")])
+    """the numeric lines (the MHD header runs over two lines)"""
+    rows = []
+    for line in text.splitlines():
+        try:
+            rows.append([float(v) for v in line.split()])
+        except ValueError:
+            continue
+    return np.array([r for r in rows if r])
 
 
 @pytest.mark.skipif(not os.path.exists(BIN), reason="binding harness not built (needs the reference tree)")
