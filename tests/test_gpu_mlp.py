@@ -212,8 +212,12 @@ def test_trained_seed_on_golden_boxes(name):
     h1 = sim.get("hh")
     assert np.max(np.abs(h1 - ref["hh"]) / ref["hh"]) <= 2 * o.tolh
     heval = _check_seeded_step(o, st, sim, 0.0, name)
-    assert np.max(np.abs(heval - ref["hh"]) / ref["hh"]) <= 2 * o.tolh
-    if "ireal" in st:
+    if "ireal" not in st:
+        # with fixed walls the first evaluation has given the fixed particles
+        # their partners' masses (copy_particle, src/conservative2primitive.f90:
+        # 425-432), so the state of the second differs from the golden input
+        assert np.max(np.abs(heval - ref["hh"]) / ref["hh"]) <= 2 * o.tolh
+    else:
         fixed = st["itype"] == 1
         np.testing.assert_array_equal(heval[fixed], heval[st["ireal"][fixed]])
 
