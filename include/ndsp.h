@@ -245,6 +245,15 @@ int ndsp_mlp_set_params(ndsp_ctx *ctx, const float *params, int32_t n);
 int ndsp_mlp_get_params(ndsp_ctx *ctx, float *params, int32_t n, int32_t *adam_steps);
 /* forward of B feature rows X[B][6] with the BF16 weights */
 int ndsp_mlp_forward(ndsp_ctx *ctx, const float *X, int32_t B, float *y);
+/* ndsp_mlp_forward that allocates nothing: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  X and y
+ * (host or device memory) are staged through a fixed device array of
+ * 65536 rows per GPU, held until the process ends; B > 65536 is rejected
+ * (NDSP_ERR_BAD_ARG), as is a context whose weights were never set
+ * (ndsp_mlp_set_params or ndsp_mlp_config first).  Calls that overlap on one
+ * GPU share that array and race. */
+int ndsp_mlp_forward_noalloc(ndsp_ctx *ctx, const float *X, int32_t B, float *y);
 /* gradient of sum (y - T)^2 / B over a caller batch (FP32 accumulation) */
 int ndsp_mlp_grad(ndsp_ctx *ctx, const float *X, const float *T, int32_t B, float *grad,
                   double *loss);

@@ -35,6 +35,7 @@
 #include <cstdlib>
 #include <initializer_list>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -2710,6 +2711,7 @@ static double nexp_of(const Phys& p) {
 }
 
 struct ndsp_ctx {
+  int device = 0;           // HIP device current at creation (holds every buffer)
   int tb_force = 256, tb_nr = 256;  // workgroup sizes of the pair kernels (autotuned)
   int nfixed = 0;           // fixed boundary particles uploaded
   bool ireal_set = true;    // their partners have been given
@@ -3099,6 +3101,36 @@ extern "C" {
 
 const char* ndsp_last_error(void) { return g_err.c_str(); }
 
+// ---------------------------------------------------------------------------
+// Fixed device staging of the allocation-free MLP inference
+// (ndsp_mlp_forward_noalloc): NOALLOC_MAX_ROWS feature rows of MLP_NIN
+// floats, then one output float per row.  A file-scope device array exists
+// once per GPU (each holds its own copy until the process ends); its address
+// on a device is taken once, with that device current, when the first
+// context is created there, and only that address is used on the host.
+// ---------------------------------------------------------------------------
+#define NOALLOC_MAX_ROWS (1 << 16)
+#define NOALLOC_MAX_DEVICES 64
+static __device__ __align__(256) float scratch[(size_t)NOALLOC_MAX_ROWS * (MLP_NIN + 1)];
+static float* g_scratch_ptr[NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+static int noalloc_scratch_init(int* device) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return set_err(NDSP_ERR_NO_DEVICE, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return set_err(NDSP_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<float*>(ptr);
+  }
+  *device = dev;
+  return 0;
+}
+
 int ndsp_create(const int32_t* iopt, const double* dopt, int64_t npart, int32_t precision, ndsp_ctx** out) {
   if (!iopt || !dopt || !out || npart < 1 || npart > 1000000000LL)
     return set_err(NDSP_ERR_BAD_ARG, "ndsp_create: bad arguments");
@@ -3166,7 +3198,10 @@ int ndsp_create(const int32_t* iopt, const double* dopt, int64_t npart, int32_t 
   p.dq2table = dq2;
   p.ddq2table = 1.0 / dq2;
 
+  int device = 0;
+  if (int e = noalloc_scratch_init(&device)) return e;
   ndsp_ctx* c = new ndsp_ctx();
+  c->device = device;
   c->p = p;
   c->prec = precision;
   c->n = (int)npart;
@@ -4362,6 +4397,28 @@ int ndsp_mlp_forward(ndsp_ctx* c, const float* X, int32_t B, float* y) {
   HIPCHK(hipMemcpyAsync(y, dy, sizeof(float) * B, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   (void)hipFree(dy);
+  return 0;
+}
+
+// ndsp_mlp_forward without any allocation: X and y go through the fixed
+// device staging above (host or device memory, hipMemcpyDefault); the
+// weights are the context's, which ndsp_mlp_set_params or ndsp_mlp_config
+// must have set.  Calls that overlap on one GPU share that staging and race.
+int ndsp_mlp_forward_noalloc(ndsp_ctx* c, const float* X, int32_t B, float* y) {
+  if (!c || !X || !y || B < 1) return set_err(NDSP_ERR_BAD_ARG, "mlp_forward_noalloc: bad arguments");
+  if (B > NOALLOC_MAX_ROWS)
+    return set_err(NDSP_ERR_BAD_ARG, "mlp_forward_noalloc: at most 65536 rows per call");
+  if (!c->mlp_alloced || !c->mlp.have_params)
+    return set_err(NDSP_ERR_BAD_ARG, "mlp_forward_noalloc: set the weights first (ndsp_mlp_set_params)");
+  if (c->device < 0 || c->device >= NOALLOC_MAX_DEVICES || !g_scratch_ptr[c->device])
+    return set_err(NDSP_ERR_HIP, "mlp_forward_noalloc: no staging address for the context's device");
+  float* dX = g_scratch_ptr[c->device];
+  float* dy = dX + (size_t)NOALLOC_MAX_ROWS * MLP_NIN;
+  HIPCHK(hipMemcpyAsync(dX, X, sizeof(float) * MLP_NIN * (size_t)B, hipMemcpyDefault, c->st));
+  mlp_launch_forward(c->mlp, dX, MLP_NIN, B, dy, c->st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(y, dy, sizeof(float) * (size_t)B, hipMemcpyDefault, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
   return 0;
 }
 
