@@ -30,13 +30,17 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
 # algorithmic compulsory bytes per particle and launch, reference layout
-# (8 B per real, each field read once and each output written once)
-BYTES_FORCE = 280        # 20 reals in, 15 reals out
+# (8 B per real, 4 B per integer, each field read once and each output
+# written once; neighbour re-reads, lists and sort scratch not counted)
+BYTES_BIN = 28           # binning (cell lists): x in, permutation and cell ranges out
 BYTES_DENSITY = 108      # per active particle and h-rho pass
-BYTES_NBLIST = 40        # neighbour lists: x, h in; count and h cap out
-BYTES_NBLIST_ENTRY = 4   # per list entry, at least numneigh of them
-FLOPS_FORCE_PAIR = 200   # FP32 flops per ordered pair evaluation (estimate, see DESIGN)
+BYTES_FORCE = 280        # 20 reals in, 15 reals out
+# algorithmic FP32 flops per ordered pair evaluation (estimates from the
+# operation counts of the pair sums: add, mul = 1, fma = 2)
+FLOPS_DENSITY_PAIR = 40
+FLOPS_FORCE_PAIR = 200
 
 
 def log(*a):
@@ -253,9 +257,20 @@ def cpu_baseline(sample_nside):
         t = time.perf_counter() - t0
         kind = "port"
         what = "CPU oracle derivs (C restatement, serial)"
-    return dict(value=pairs / t, unit="pair interactions/s", cores=1, kind=kind,
+    return dict(value=pairs / t, unit="pair interactions/s", cores=1, kind=kind, cpu=cpu_model(),
                 sample="%s on the %d^3 box (%d particles, %d pairs, %.2f s)" % (
                     what, sample_nside, sample_nside ** 3, pairs, t))
+
+
+def cpu_model():
+    """the host CPU's model name (first processor in /proc/cpuinfo)"""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def main():
@@ -298,20 +313,28 @@ def main():
     ms_step = 1e3 * r["wall"] / args.steps
     kt = r["kernel_ms"]
     log("[bench] %.3f ms/step, %.3e pairs/s, kernels %s" % (ms_step, value, kt))
-    pair_kernels = {k: kt[k] for k in ("nblist", "density", "force") if k in kt}
-    dom = max(pair_kernels, key=pair_kernels.get) if pair_kernels else "force"
-    force_ms = kt.get("force", float("nan"))
-    # algorithmic bytes per launch of the dominant kernel
-    if dom == "density":
-        alg_bytes = BYTES_DENSITY * n * r["ncalc_mean"]
-    elif dom == "nblist":
-        alg_bytes = (BYTES_NBLIST + BYTES_NBLIST_ENTRY * r["numneigh_mean"]) * n
-    else:
-        alg_bytes = BYTES_FORCE * n
-    dom_ms = kt.get(dom, float("nan"))
-    achieved = alg_bytes / (dom_ms * 1e-3) / 1e9
     pairs_force_step = r["pairs_force"] / args.steps
-    flops = FLOPS_FORCE_PAIR * 2 * pairs_force_step
+    pairs_density_step = r["pairs_density"] / args.steps
+    # roofline of each pair kernel: algorithmic bytes and FP32 flops per
+    # launch; t_roof = max(bytes / HBM peak, flops / FP32 peak)
+    work = {"nblist": (BYTES_BIN * n, 0.0),
+            "density": (BYTES_DENSITY * n * r["ncalc_mean"], FLOPS_DENSITY_PAIR * pairs_density_step),
+            "force": (BYTES_FORCE * n, FLOPS_FORCE_PAIR * 2 * pairs_force_step)}
+    per_kernel = {}
+    for k, (b, f) in work.items():
+        if k not in kt:
+            continue
+        t = kt[k] * 1e-3
+        t_roof = max(b / (HBM_PEAK_GBS * 1e9), f / (FP32_PEAK_TFLOPS * 1e12))
+        per_kernel[k] = {"ms": kt[k], "alg_bytes": b, "alg_fp32_flops": f,
+                         "hbm_gbs": b / t / 1e9, "hbm_frac": b / t / 1e9 / HBM_PEAK_GBS,
+                         "fp32_tflops": f / t / 1e12, "fp32_frac": f / t / 1e12 / FP32_PEAK_TFLOPS,
+                         "roof_frac": t_roof / t,
+                         "traffic": traffic_from_profile({"density": "k_nr_pass"}.get(k, "k_" + k))}
+    dom = max(per_kernel, key=lambda k: per_kernel[k]["ms"]) if per_kernel else "force"
+    alg_bytes = per_kernel[dom]["alg_bytes"]
+    dom_ms = per_kernel[dom]["ms"]
+    achieved = per_kernel[dom]["hbm_gbs"]
     result = {
         "metric": "pair interactions/s (3D 128^3 SPMHD leapfrog step, density + force pairs)",
         "value": value,
@@ -324,15 +347,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 pair sums (compensated), f64 positions/integrator/reductions",
+        "dtype": "f32 pair math with f64 accumulation; f64 positions, neighbour tests, h-rho update, integrator, reductions",
         "data": "synthetic",
         "config": {"workload": "3D periodic MHD box %d^3 (%d particles), cubic spline, hfact 1.2, "
                                "gamma 5/3, v_rms Mach 1 P(k)~k^-4, B=(0.1,0,0)" % (args.nside, n),
                    "particles": n, "steps": args.steps},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile({"density": "k_nr_pass"}.get(dom, "k_" + dom)),
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": per_kernel[dom]["traffic"],
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": dom_ms,
-                     "force_fp32_tflops": flops / (force_ms * 1e-3) / 1e12 if force_ms == force_ms else None},
+                     "per_kernel": per_kernel},
         "kernel_ms": kt,
         "tile_shapes": r["shapes"],
         "numneigh_mean": r["numneigh_mean"],
@@ -362,6 +386,15 @@ def main():
                               "steps_per_s": args.steps / r64["wall"],
                               "speedup_mixed_over_fp64": (args.steps / rg["wall"]) / (args.steps / r64["wall"])}
         r64["sim"].close()
+        # the same steps in FP64 with an event around every kernel group:
+        # where the mixed path gains, kernel by kernel
+        log("[bench] plain FP64 GPU, kernel times")
+        r64t = run_gpu(args.nside, args.steps, args.warmup, prec=1, timing=True, graph=False)
+        kt64 = r64t["kernel_ms"]
+        result["fp64_gpu"]["kernel_ms"] = kt64
+        result["fp64_gpu"]["fp64_over_mixed_per_kernel"] = {
+            k: kt64[k] / kt[k] for k in kt if k in kt64 and kt[k] > 0}
+        r64t["sim"].close()
         result["error_vs_fp64"] = field_errors(min(args.nside, 64))
     if args.big_nside > 0 and not args.no_fp64:
         nb = args.big_nside

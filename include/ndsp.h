@@ -194,8 +194,11 @@ int ndsp_set_tile_shapes(ndsp_ctx *ctx, int32_t force_wg, int32_t density_wg,
                          int32_t cells_per_radius);
 
 /* Test support (no counterpart in the reference): copies nbytes between two
- * fresh device buffers with the library's plain copy kernel, a reference of
- * known traffic for calibrating hardware memory counters. */
+ * fresh device buffers with the library's plain copy kernel (k_copy_d), then
+ * reads the largest power-of-two number of 64-byte lines that fits in nbytes
+ * once each, in a scattered order (k_gather_probe, 4 bytes written per
+ * line): references of known traffic for calibrating hardware memory
+ * counters on streaming and on gather-shaped access. */
 int ndsp_copy_probe(ndsp_ctx *ctx, int64_t nbytes);
 
 /* Step graphs (no counterpart in the reference, whose step is a host loop):

@@ -280,7 +280,8 @@ class Simulation:
                                                          int(cells_per_radius)))
 
     def copy_probe(self, nbytes: int):
-        """plain device copy of nbytes (counter calibration, ndsp_copy_probe)"""
+        """plain device copy of nbytes, then a scattered read of as many
+        64-byte lines (counter calibration, ndsp_copy_probe)"""
         self._check(load_library().ndsp_copy_probe(self.h, int(nbytes)))
 
     def set_reorder(self, every: int):

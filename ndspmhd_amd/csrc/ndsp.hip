@@ -2411,6 +2411,17 @@ __global__ void k_copy_d(const double* __restrict__ in, double* __restrict__ out
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) out[k] = in[k];
 }
+// counter calibration for gather-shaped access: thread i reads the 64-byte
+// line (i * mult) mod nl of src (a bijection for odd mult, nl a power of
+// two: every line once, in a scattered order) and writes one float
+__global__ void k_gather_probe(const double4* __restrict__ src, float* __restrict__ out, long long nl,
+                               unsigned long long mult) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nl) return;
+  const long long j = (long long)(((unsigned long long)i * mult) & (unsigned long long)(nl - 1));
+  const double4 u = src[2 * j], v = src[2 * j + 1];
+  out[i] = (float)(u.x + u.y + u.z + u.w + v.x + v.y + v.z + v.w);
+}
 __global__ void k_copy_i(const int* __restrict__ in, int* __restrict__ out, int n) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) out[k] = in[k];
@@ -3416,6 +3427,21 @@ int ndsp_download(ndsp_ctx* c, const char* name, double* out, int32_t* width) {
     if (width) *width = 1;
     return 0;
   }
+  if (!strcmp(name, "nblist_len")) {
+    // entries of each particle's neighbour list at the last binning
+    // (gather list + extras); -1 where it scans cells instead (an overflow,
+    // or lists off)
+    std::vector<int> nbc(n, -1), nbx(n, 0), sidx(n);
+    if (c->p.use_list) {
+      HIPCHK(hipMemcpy(nbc.data(), c->a.nbc, sizeof(int) * n, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(nbx.data(), c->a.nbx, sizeof(int) * n, hipMemcpyDeviceToHost));
+    }
+    HIPCHK(hipMemcpy(sidx.data(), c->a.sidx, sizeof(int) * n, hipMemcpyDeviceToHost));
+    for (int k = 0; k < n; k++)
+      out[uid[sidx[k]]] = nbc[k] < 0 || nbc[k] + nbx[k] > c->a.maxnb ? -1.0 : (double)(nbc[k] + nbx[k]);
+    if (width) *width = 1;
+    return 0;
+  }
   if (!strcmp(name, "numneigh") || !strcmp(name, "nits")) {
     std::vector<int> ib(n);
     HIPCHK(hipMemcpy(ib.data(), !strcmp(name, "nits") ? c->a.nits : c->a.numneigh, sizeof(int) * n,
@@ -4182,6 +4208,16 @@ int ndsp_copy_probe(ndsp_ctx* c, int64_t nbytes) {
   HIPCHK(hipStreamSynchronize(c->st));
   hipLaunchKernelGGL(k_copy_d, dim3(nblk(nd)), dim3(BS), 0, c->st, (const double*)src, dst, (int)nd);
   HIPCHK(hipStreamSynchronize(c->st));
+  // the same bytes read as scattered 64-byte lines (gather-shaped)
+  long long nl = 1;
+  while (nl * 2 * 64 <= (long long)nd * 8) nl *= 2;
+  if (nl >= 64) {
+    hipLaunchKernelGGL(k_fill_d, dim3(4096), dim3(BS), 0, c->st, dst, (int)(nl * 8), 2.0);
+    HIPCHK(hipStreamSynchronize(c->st));
+    hipLaunchKernelGGL(k_gather_probe, dim3((unsigned)((nl + BS - 1) / BS)), dim3(BS), 0, c->st,
+                       (const double4*)dst, (float*)src, nl, 0x9E3779B97F4A7C15ull);
+    HIPCHK(hipStreamSynchronize(c->st));
+  }
   hipFree(src);
   hipFree(dst);
   return 0;

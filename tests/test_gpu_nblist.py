@@ -78,3 +78,31 @@ def test_list_abandoned_when_h_outgrows_skin():
         res.append(sim)
     assert np.array_equal(res[0].get("numneigh"), res[1].get("numneigh"))
     _close(res[0], res[1], 1e-12)
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-12), (0, 1e-6)])
+def test_lists_fall_back_where_they_overflow(prec, tol):
+    """a few particles with h 2.5 times their neighbours' at fixed h: their
+    groups' candidates exceed the staging capacity and their rows the list
+    capacity, so those particles fall back to the cell scan; the results
+    equal the scan's"""
+    from ndspmhd_amd.setups import periodic_mhd_box
+    o, st = periodic_mhd_box(16, seed=7, vel_seed=8, vrms=0.3)
+    rng = np.random.default_rng(9)
+    big = rng.choice(st["hh"].size, 40, replace=False)
+    st["hh"][big] *= 2.5
+    o = o.replace(maxdensits=0)
+    print("mhd box 16^3 with 40 particles of 2.5 h, prec", prec, flush=True)
+    import ndspmhd_amd as nd
+    probe = nd.Simulation(o, st, precision=prec)
+    probe.derivs()
+    L = probe.get("nblist_len")
+    nn = probe.get("numneigh")
+    print("list fallbacks", int((L < 0).sum()), "of", L.size, flush=True)
+    assert (L < 0).any() and (L >= 0).mean() > 0.5
+    # a list holds at least the symmetric neighbours of its particle
+    assert np.all(L[L >= 0] >= nn[L >= 0])
+    (s0, n0, c0), (s1, n1, c1) = _pair(o, st, prec, 2)
+    assert "nblist" in n0
+    assert np.array_equal(c0[0], c1[0])
+    _close(s0, s1, tol)
