@@ -248,6 +248,10 @@ int ndsp_mlp_set_params(ndsp_ctx *ctx, const float *params, int32_t n);
 int ndsp_mlp_get_params(ndsp_ctx *ctx, float *params, int32_t n, int32_t *adam_steps);
 /* forward of B feature rows X[B][6] with the BF16 weights */
 int ndsp_mlp_forward(ndsp_ctx *ctx, const float *X, int32_t B, float *y);
+/* the same forward on the matrix cores (BF16 inputs and activations, FP32
+ * accumulation), the arithmetic of the seed the steps use; no counterpart in
+ * the reference */
+int ndsp_mlp_infer(ndsp_ctx *ctx, const float *X, int32_t B, float *y);
 /* ndsp_mlp_forward that allocates nothing: no counterpart in the reference;
  * for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  X and y

@@ -36,7 +36,7 @@ SYMBOLS = ("ndsp_create", "ndsp_destroy", "ndsp_upload", "ndsp_download",
            "ndsp_derivs", "ndsp_step", "ndsp_set_dt", "ndsp_conserved", "ndsp_ev_columns",
            "ndsp_neighbours", "ndsp_set_timing", "ndsp_kernel_times", "ndsp_set_graph",
            "ndsp_mlp_config", "ndsp_mlp_set_params", "ndsp_mlp_get_params",
-           "ndsp_mlp_forward", "ndsp_mlp_forward_noalloc", "ndsp_mlp_grad", "ndsp_mlp_train",
+           "ndsp_mlp_forward", "ndsp_mlp_infer", "ndsp_mlp_forward_noalloc", "ndsp_mlp_grad", "ndsp_mlp_train",
            "ndsp_batch_create", "ndsp_batch_destroy", "ndsp_batch_step", "ndsp_batch_autotune", "ndsp_autotune",
            "ndsp_set_reorder", "ndsp_set_ireal", "ndsp_set_tile_shapes",
            "ndsp_copy_probe",
@@ -104,6 +104,7 @@ def load_library(path: str | None = None):
     L.ndsp_mlp_get_params.argtypes = [vp, fp, ctypes.c_int32, ip]
     L.ndsp_mlp_forward.argtypes = [vp, fp, ctypes.c_int32, fp]
     L.ndsp_mlp_forward_noalloc.argtypes = [vp, fp, ctypes.c_int32, fp]
+    L.ndsp_mlp_infer.argtypes = [vp, fp, ctypes.c_int32, fp]
     L.ndsp_mlp_grad.argtypes = [vp, fp, fp, ctypes.c_int32, fp, dp]
     L.ndsp_mlp_train.argtypes = [vp, fp, fp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, dp]
     L.ndsp_last_error.restype = ctypes.c_char_p
@@ -246,6 +247,13 @@ class Simulation:
         X = np.ascontiguousarray(X, dtype=np.float32)
         y = np.empty(X.shape[0], dtype=np.float32)
         self._check(load_library().ndsp_mlp_forward(self.h, _fp(X), X.shape[0], _fp(y)))
+        return y
+
+    def mlp_infer(self, X):
+        """the seed's inference arithmetic (matrix cores, BF16 activations)"""
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.empty(X.shape[0], dtype=np.float32)
+        self._check(load_library().ndsp_mlp_infer(self.h, _fp(X), X.shape[0], _fp(y)))
         return y
 
     def mlp_grad(self, X, T):

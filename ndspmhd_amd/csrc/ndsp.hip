@@ -2508,22 +2508,6 @@ __global__ void k_nb(KArgs a, Phys p, const Grid* __restrict__ gp, int sym, cons
 // h-predictor features and targets (MLP seed of the h-rho iteration)
 // ===========================================================================
 
-// after the predictor: features of each particle and the predictor's h
-__global__ void k_mlp_features(KArgs a, Phys p, double dt, double nexp, float* __restrict__ X) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
-  double** f = a.f;
-  const double hin = f[F_HIN][i], rho = f[F_RHO][i], m = f[F_M][i];
-  float* x = X + (size_t)i * MLP_LDX;
-  x[0] = (float)(100.0 * dt * f[F_DHDTIN][i] / hin);
-  x[1] = rho > 0.0 ? (float)(100.0 * dt * f[F_DRHODT][i] / rho) : 0.0f;
-  x[2] = (float)(1000.0 * f[F_MLPCORR][i]);
-  x[3] = (float)((a.numneigh[i] - nexp) / nexp);
-  x[4] = (float)(a.nits[i] - 1.5);
-  x[5] = rho > 0.0 ? (float)(1000.0 * (hin * pow(rho / m, p.dndim) / p.hfact - 1.0)) : 0.0f;
-  x[MLP_COL_HPRED] = (float)f[F_H][i];
-}
-
 // after the h-rho iteration: target log(h_converged / h_predictor)
 __global__ void k_mlp_target(KArgs a, float* __restrict__ X) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -3524,11 +3508,30 @@ static int step_part_a(ndsp_ctx* c) {
   }
   MlpState& ml = c->mlp;
   if (ml.enabled) {
+    // features of every particle (the training rows) and, once the network
+    // has trained for seed_after steps, the seed itself, in one launch on
+    // the matrix cores
     TScope t(c, "mlp_seed");
-    hipLaunchKernelGGL(k_mlp_features, dim3(nblk(c->n)), dim3(BS), 0, c->st, c->a, c->p, c->dt, nexp_of(c->p),
-                       ml.X);
-    if (ml.have_params && ml.steps_seen >= ml.seed_after)
-      mlp_launch_seed(ml, c->n, c->a.f[F_H], c->p.fixed_on ? c->a.itype : nullptr, c->st);
+    MlpSeedFields f{};
+    double** fd = c->a.f;
+    f.hin = fd[F_HIN];
+    f.rho = fd[F_RHO];
+    f.m = fd[F_M];
+    f.dhdtin = fd[F_DHDTIN];
+    f.drhodt = fd[F_DRHODT];
+    f.corr = fd[F_MLPCORR];
+    f.numneigh = c->a.numneigh;
+    f.nits = c->a.nits;
+    f.itype = c->p.fixed_on ? c->a.itype : nullptr;
+    f.h = fd[F_H];
+    f.X = ml.X;
+    f.dt = c->dt;
+    f.nexp = nexp_of(c->p);
+    f.hfact = c->p.hfact;
+    f.dndim = c->p.dndim;
+    f.n = c->n;
+    f.apply = ml.have_params && ml.steps_seen >= ml.seed_after;
+    mlp_launch_seed_mfma(ml, f, c->st);
   }
   if (int e = do_linklist(c, 1)) return e;
   if (int e = do_density(c)) return e;
@@ -4429,6 +4432,20 @@ int ndsp_mlp_forward(ndsp_ctx* c, const float* X, int32_t B, float* y) {
   float* dy = nullptr;
   HIPCHK(hipMalloc(&dy, sizeof(float) * B));
   mlp_launch_forward(c->mlp, c->mlp.Xext, MLP_LDX, B, dy, c->st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(y, dy, sizeof(float) * B, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  (void)hipFree(dy);
+  return 0;
+}
+
+int ndsp_mlp_infer(ndsp_ctx* c, const float* X, int32_t B, float* y) {
+  if (!c || !X || !y || B < 1) return set_err(NDSP_ERR_BAD_ARG, "mlp_infer: bad arguments");
+  if (int e = mlp_ready(c)) return e;
+  if (int e = mlp_stage(c, X, nullptr, B)) return e;
+  float* dy = nullptr;
+  HIPCHK(hipMalloc(&dy, sizeof(float) * B));
+  mlp_launch_infer(c->mlp, c->mlp.Xext, MLP_LDX, B, dy, c->st);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(y, dy, sizeof(float) * B, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));

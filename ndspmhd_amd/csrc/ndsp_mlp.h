@@ -33,11 +33,32 @@ struct MlpState {
   double last_loss = 0.0;
 };
 
+// the particle fields the seed's features are formed from (one evaluation
+// after the predictor); h is scaled in place when apply is set, X receives
+// the feature rows (6 features, predictor h) when not null
+struct MlpSeedFields {
+  const double* hin;
+  const double* rho;
+  const double* m;
+  const double* dhdtin;
+  const double* drhodt;
+  const double* corr;
+  const int* numneigh;
+  const int* nits;
+  const int* itype;  // null: no fixed particles
+  double* h;
+  float* X;
+  double dt, nexp, hfact, dndim;
+  int n, apply;
+};
+
 hipError_t mlp_alloc(MlpState& s, int n);
 void mlp_free(MlpState& s);
 hipError_t mlp_set_params(MlpState& s, const float* host, hipStream_t st);
-void mlp_launch_seed(MlpState& s, int n, double* h, const int* itype, hipStream_t st);
 void mlp_launch_forward(MlpState& s, const float* X, int ldx, int B, float* y, hipStream_t st);
+// the same network on the matrix cores (BF16 inputs and activations)
+void mlp_launch_infer(MlpState& s, const float* X, int ldx, int B, float* y, hipStream_t st);
+void mlp_launch_seed_mfma(MlpState& s, const MlpSeedFields& f, hipStream_t st);
 void mlp_launch_grad(MlpState& s, const float* X, int nrows, int B, long long off, long long stride,
                      hipStream_t st);
 void mlp_launch_adam(MlpState& s, float lr, hipStream_t st);

@@ -80,25 +80,6 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const uint16_t* __restrict_
   y[b] = mlp_fwd(sw, x, z1, a1, z2, a2);
 }
 
-// seed of the h-rho iteration: h = h_pred exp(y / scale), |y/scale| <= 0.1,
-// applied to the predictor's FP64 h in place (a zero correction leaves it
-// unchanged); fixed boundary particles (itype 1, when given) keep the h the
-// predictor reset them to (src/stepND_leapfrog_mhd.f90:136)
-__global__ __launch_bounds__(256) void k_mlp_seed(const uint16_t* __restrict__ wbf, const float* __restrict__ X,
-                                                  int n, double* __restrict__ h, const int* __restrict__ itype) {
-  __shared__ float sw[MLP_NPARAM];
-  load_weights(wbf, sw);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (itype && itype[i] == 1) return;
-  float x[NIN], z1[NH], a1[NH], z2[NH], a2[NH];
-  for (int q = 0; q < NIN; q++) x[q] = X[(size_t)i * MLP_LDX + q];
-  float y = mlp_fwd(sw, x, z1, a1, z2, a2) / MLP_TSCALE;
-  y = fminf(fmaxf(y, -0.1f), 0.1f);
-  const double hpred = h[i];
-  if (isfinite(y) && y != 0.0f && hpred > 0.0) h[i] = hpred * exp((double)y);
-}
-
 // gradient of L = sum (y - t)^2 / B over the samples idx(b), b < B; sample b
 // is row (off + b * stride) mod nrows of X (target in column tcol)
 __global__ __launch_bounds__(WB) void k_mlp_grad(const uint16_t* __restrict__ wbf, const float* __restrict__ X,
@@ -195,6 +176,143 @@ __global__ void k_mlp_to_bf16(const float* __restrict__ w, uint16_t* __restrict_
 }
 
 // ---------------------------------------------------------------------------
+// Inference on the matrix cores (v_mfma_f32_32x32x8bf16_1k): BF16 inputs and
+// activations, FP32 accumulation.  A wavefront takes 64 samples as two
+// blocks of 32.  Each layer is computed transposed, Z^T = W A^T: the weights
+// are the A operand (row = output unit = lane % 32, four inputs 4 (lane / 32)
+// + j per lane), the activations the B operand (column = sample = lane % 32,
+// four inputs 4 (lane / 32) + j per lane), and the result D holds, in its
+// register v, output unit 8 (v / 4) + 4 (lane / 32) + v % 4 of sample
+// lane % 32 -- registers 4s..4s+3 are then exactly the B operand of the next
+// layer's K-step s, so activations never leave the registers.  Layer 1 pads
+// the 6 features to K = 8; layer 2 is four K-steps; layer 3 (32 -> 1) is a
+// dot product per lane over its 16 units plus one exchange between the two
+// half-waves.
+// ---------------------------------------------------------------------------
+typedef short mlp_bf16x4 __attribute__((ext_vector_type(4)));
+typedef float mlp_f32x16 __attribute__((ext_vector_type(16)));
+
+// tanh-form GELU with the hardware exponential and reciprocal
+__device__ __forceinline__ float gelu_fast(float x) {
+  const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
+  const float e = __expf(-2.0f * fabsf(u));
+  const float t = (1.0f - e) * __builtin_amdgcn_rcpf(1.0f + e);
+  return 0.5f * x * (1.0f + copysignf(t, u));
+}
+
+// per-lane weights of the transposed layers (see above)
+struct MlpLaneW {
+  mlp_bf16x4 a1, a2[4];
+  float b1[16], b2[16], w3[16];
+  float b3;
+};
+__device__ __forceinline__ void mlp_lane_weights(const uint16_t* __restrict__ wbf, MlpLaneW& w) {
+  const int lane = threadIdx.x & 63, i = lane & 31, hb = lane >> 5;
+  const uint16_t* W1 = wbf;
+  const uint16_t* B1 = W1 + NH * NIN;
+  const uint16_t* W2 = B1 + NH;
+  const uint16_t* B2 = W2 + NH * NH;
+  const uint16_t* W3 = B2 + NH;
+  for (int j = 0; j < 4; j++) {
+    const int kin = 4 * hb + j;
+    w.a1[j] = kin < NIN ? (short)W1[i * NIN + kin] : (short)0;
+    for (int s = 0; s < 4; s++) w.a2[s][j] = (short)W2[i * NH + 8 * s + 4 * hb + j];
+  }
+  for (int v = 0; v < 16; v++) {
+    const int u = 8 * (v >> 2) + 4 * hb + (v & 3);
+    w.b1[v] = bf16_to_f(B1[u]);
+    w.b2[v] = bf16_to_f(B2[u]);
+    w.w3[v] = bf16_to_f(W3[u]);
+  }
+  w.b3 = bf16_to_f(W3[NH]);
+}
+
+// the network on one block of 32 samples: x0..x3 are the lane's four inputs
+// (features 4 (lane / 32) + j of sample lane % 32); returns the output of
+// sample lane % 32 in both half-waves
+__device__ __forceinline__ float mlp_mfma_block(const MlpLaneW& w, const float (&x)[4]) {
+  mlp_bf16x4 b;
+  for (int j = 0; j < 4; j++) b[j] = (short)f_to_bf16(x[j]);
+  mlp_f32x16 zero;
+  for (int v = 0; v < 16; v++) zero[v] = 0.0f;
+  const mlp_f32x16 d1 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w.a1, b, zero, 0, 0, 0);
+  mlp_bf16x4 h[4];
+  for (int v = 0; v < 16; v++) h[v >> 2][v & 3] = (short)f_to_bf16(gelu_fast(d1[v] + w.b1[v]));
+  mlp_f32x16 d2 = zero;
+  for (int s = 0; s < 4; s++) d2 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w.a2[s], h[s], d2, 0, 0, 0);
+  float part = 0.0f;
+  for (int v = 0; v < 16; v++) part += w.w3[v] * gelu_fast(d2[v] + w.b2[v]);
+  return part + __shfl_xor(part, 32) + w.b3;
+}
+
+// inference on caller rows X[B][ldx] (features in columns 0..5)
+__global__ __launch_bounds__(256) void k_mlp_infer(const uint16_t* __restrict__ wbf, const float* __restrict__ X,
+                                                   int ldx, int B, float* __restrict__ y) {
+  MlpLaneW w;
+  mlp_lane_weights(wbf, w);
+  const int lane = threadIdx.x & 63, i = lane & 31, hb = lane >> 5;
+  const long long base = ((long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63)) ;
+  for (int r = 0; r < 2; r++) {
+    const long long p = base + 32 * r + i;
+    float x[4] = {0.f, 0.f, 0.f, 0.f};
+    if (p < B)
+      for (int j = 0; j < 4; j++) {
+        const int k = 4 * hb + j;
+        if (k < NIN) x[j] = X[p * ldx + k];
+      }
+    const float out = mlp_mfma_block(w, x);
+    if (hb == 0 && p < B) y[p] = out;
+  }
+}
+
+// the seed of the h-rho iteration after the predictor: features formed from
+// the particle fields, optionally stored as training rows, and (apply) the
+// predictor's h scaled by exp(y / MLP_TSCALE), |y / MLP_TSCALE| <= 0.1;
+// fixed boundary particles (itype 1) keep the h the predictor gave them
+__global__ __launch_bounds__(256) void k_mlp_seed_mfma(const uint16_t* __restrict__ wbf, MlpSeedFields f) {
+  MlpLaneW w;
+  mlp_lane_weights(wbf, w);
+  const int lane = threadIdx.x & 63, i = lane & 31, hb = lane >> 5;
+  const long long base = ((long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63));
+  for (int r = 0; r < 2; r++) {
+    const long long p = base + 32 * r + i;
+    const bool ok = p < f.n;
+    float x[4] = {0.f, 0.f, 0.f, 0.f};
+    double hpred = 0.0;
+    if (ok) {
+      const double hin = f.hin[p], rho = f.rho[p];
+      hpred = f.h[p];
+      if (hb == 0) {
+        x[0] = (float)(100.0 * f.dt * f.dhdtin[p] / hin);
+        x[1] = rho > 0.0 ? (float)(100.0 * f.dt * f.drhodt[p] / rho) : 0.0f;
+        x[2] = (float)(1000.0 * f.corr[p]);
+        x[3] = (float)((f.numneigh[p] - f.nexp) / f.nexp);
+      } else {
+        x[0] = (float)(f.nits[p] - 1.5);
+        x[1] = rho > 0.0 ? (float)(1000.0 * (hin * pow(rho / f.m[p], f.dndim) / f.hfact - 1.0)) : 0.0f;
+      }
+    }
+    const float out = mlp_mfma_block(w, x);
+    if (!ok) continue;
+    if (f.X) {
+      float* row = f.X + p * MLP_LDX;
+      if (hb == 0) {
+        for (int j = 0; j < 4; j++) row[j] = x[j];
+      } else {
+        row[4] = x[0];
+        row[5] = x[1];
+        row[MLP_COL_HPRED] = (float)hpred;
+      }
+    }
+    if (f.apply && hb == 0 && !(f.itype && f.itype[p] == 1)) {
+      float yy = out / MLP_TSCALE;
+      yy = fminf(fmaxf(yy, -0.1f), 0.1f);
+      if (isfinite(yy) && yy != 0.0f && hpred > 0.0) f.h[p] = hpred * exp((double)yy);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
 
@@ -237,9 +355,13 @@ hipError_t mlp_set_params(MlpState& s, const float* host, hipStream_t st) {
   return hipGetLastError();
 }
 
-void mlp_launch_seed(MlpState& s, int n, double* h, const int* itype, hipStream_t st) {
-  hipLaunchKernelGGL(k_mlp_seed, dim3(nb(n, 256)), dim3(256), 0, st, (const uint16_t*)s.wbf, (const float*)s.X, n, h,
-                     itype);
+
+void mlp_launch_seed_mfma(MlpState& s, const MlpSeedFields& f, hipStream_t st) {
+  hipLaunchKernelGGL(k_mlp_seed_mfma, dim3(nb(f.n, 256)), dim3(256), 0, st, (const uint16_t*)s.wbf, f);
+}
+
+void mlp_launch_infer(MlpState& s, const float* X, int ldx, int B, float* y, hipStream_t st) {
+  hipLaunchKernelGGL(k_mlp_infer, dim3(nb(B, 256)), dim3(256), 0, st, (const uint16_t*)s.wbf, X, ldx, B, y);
 }
 
 void mlp_launch_forward(MlpState& s, const float* X, int ldx, int B, float* y, hipStream_t st) {

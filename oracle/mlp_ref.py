@@ -9,6 +9,7 @@ same loss, computed in FP64 from the BF16-rounded weights the device uses.
 Parameter vector (float32[1313]):
   W1[32][6] b1[32] W2[32][32] b2[32] W3[32] b3[1]    (row-major, out x in)
 Forward:  a1 = gelu(W1 x + b1), a2 = gelu(W2 a1 + b2), y = W3 a2 + b3
+          (forward_bf16: the matrix-core inference, x and a1 in BF16)
 Loss:     L = sum_b (y_b - t_b)^2 / B
 Adam:     m = b1 m + (1-b1) g, v = b2 v + (1-b2) g^2,
           w -= lr * (m / (1-b1^k)) / (sqrt(v / (1-b2^k)) + eps)
@@ -61,6 +62,17 @@ def forward(p, X):
     a2 = gelu(z2)
     y = a2 @ W3 + b3[0]
     return y, (X, z1, a1, z2, a2)
+
+
+def forward_bf16(p, X):
+    """the matrix-core inference of the seed: weights, inputs and the first
+    hidden layer's activations rounded to BF16, sums in FP64 here (FP32 on
+    the device), the second hidden layer and the output unrounded"""
+    W1, b1, W2, b2, W3, b3 = unpack(bf16_round(p))
+    Xb = bf16_round(np.asarray(X, dtype=np.float32))
+    a1 = bf16_round(gelu(Xb @ W1.T + b1).astype(np.float32))
+    a2 = gelu(a1 @ W2.T + b2)
+    return a2 @ W3 + b3[0]
 
 
 def loss_and_grad(p, X, T):

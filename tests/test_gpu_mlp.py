@@ -37,6 +37,32 @@ def test_forward_and_gradient_match_oracle():
     assert np.max(np.abs(g_bad - g_ref)) / np.max(np.abs(g_ref)) > 1e-4 or abs(g_ref[200]) < 1e-3
 
 
+def test_matrix_core_inference_matches_checker():
+    """the seed's inference on the matrix cores (BF16 inputs and first-layer
+    activations, FP32 accumulation) against the checker's emulation of that
+    arithmetic; an operand placed in the wrong lane or register would give
+    errors of order one.  Against the FP32 forward it differs by the BF16
+    rounding of the activations only."""
+    from oracle import mlp_ref as m
+    sim = _sim()
+    p = m.init_params(13)
+    rng = np.random.default_rng(14)
+    p[m.NH * m.NIN:m.NH * m.NIN + m.NH] = rng.normal(0, 0.1, m.NH)
+    p[-1] = 0.3
+    X = rng.standard_normal((2000, 6)).astype(np.float32)
+    sim.mlp_set_params(p)
+    y = sim.mlp_infer(X)
+    ye = m.forward_bf16(p, X.astype(np.float64))
+    err = np.max(np.abs(y - ye)) / np.max(np.abs(ye))
+    y32 = sim.mlp_forward(X)
+    err32 = np.max(np.abs(y - y32)) / np.max(np.abs(y32))
+    print("mfma inference vs bf16 emulation %.2e, vs fp32 forward %.2e" % (err, err32), flush=True)
+    assert err < 5e-3
+    assert err32 < 5e-2
+    # the last, partial block (2000 = 31 waves + 16 rows) is complete
+    assert np.all(np.isfinite(y))
+
+
 def test_adam_steps_match_oracle():
     from oracle import mlp_ref as m
     sim = _sim()

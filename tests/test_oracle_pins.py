@@ -554,3 +554,27 @@ def test_mlp_ref_gradient_finite_difference():
     z = np.linspace(-4, 4, 101)
     fdg = (m.gelu(z + 1e-6) - m.gelu(z - 1e-6)) / 2e-6
     assert np.max(np.abs(fdg - m.dgelu(z))) < 1e-8
+
+
+def test_mlp_ref_bf16_forward_emulation():
+    """the emulation of the matrix-core inference is the forward with BF16
+    rounding of the inputs and first-layer activations: equal to it where
+    that rounding is exact (inputs and activations representable), within the
+    rounding's effect otherwise"""
+    from oracle import mlp_ref as m
+    rng = np.random.default_rng(15)
+    p = m.init_params(8)
+    p[m.NH * m.NIN:m.NH * m.NIN + m.NH] = rng.normal(0, 0.2, m.NH)
+    X = rng.standard_normal((256, m.NIN)).astype(np.float32)
+    yb = m.forward_bf16(p, X)
+    yf, _ = m.forward(m.bf16_round(p), m.bf16_round(X))
+    assert np.max(np.abs(yb - yf)) < 2e-2 * np.max(np.abs(yf))
+    assert np.max(np.abs(yb - yf)) > 0.0
+    # a network whose first layer outputs are exact in BF16: W1 = 0, b1 a
+    # BF16 value whose GELU is exact too (0): then both agree to round-off
+    q = m.bf16_round(p).copy()
+    q[:m.NH * m.NIN] = 0.0
+    q[m.NH * m.NIN:m.NH * m.NIN + m.NH] = 0.0
+    yb0 = m.forward_bf16(q, X)
+    yf0, _ = m.forward(q, X)
+    assert np.max(np.abs(yb0 - yf0)) < 1e-12
