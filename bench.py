@@ -111,21 +111,20 @@ def conservation(cons0, cons1, mass):
 
 def traffic_from_profile(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, each scaled by
-    the ratio of known bytes to counted bytes of the plain copy kernel);
-    None when no summary is committed"""
+    summary: the DRAM-side read and write request counters in 32-byte units
+    (TCC_EA0_RDREQ_DRAM_32B, TCC_EA0_WRREQ_WRITE_DRAM_32B), which read the
+    exact byte count for the plain copy kernel of the same run; None when no
+    summary is committed"""
     path = os.path.join(ROOT, "profiles", "pmc_traffic_mhd128_mixed.txt")
     if not os.path.exists(path):
         return None
     vals = {}
     for line in open(path):
         f = line.split()
-        if len(f) == 3 and f[0] in ("fetch_kib", "write_kib"):
+        if len(f) == 3 and f[0] in ("dram_read_bytes", "dram_write_bytes"):
             vals[(f[0], f[1])] = float(f[2])
     try:
-        cf = vals[("fetch_kib", "k_copy_d")] * 1024.0 / vals[("fetch_kib", "copy_bytes")]
-        cw = vals[("write_kib", "k_copy_d")] * 1024.0 / vals[("write_kib", "copy_bytes")]
-        return (vals[("fetch_kib", kernel)] * 1024.0 / cf + vals[("write_kib", kernel)] * 1024.0 / cw)
+        return vals[("dram_read_bytes", kernel)] + vals[("dram_write_bytes", kernel)]
     except KeyError:
         return None
 
@@ -154,6 +153,15 @@ def run_mlp(nside, steps):
         wall = time.perf_counter() - t0
         out[mode] = {"ncalc_mean_second_half": float(np.mean(nc[steps // 2:])), "steps_per_s": steps / wall}
         fields[mode] = (sim.get("hh").copy(), sim.get("rho").copy())
+        # per-kernel time (ms per evaluation, one evaluation a step) over a few
+        # more steps with events on: the seed's own cost beside what it saves
+        # in the density passes
+        sim.set_timing(True)
+        for _ in range(4):
+            sim.step()
+        out[mode]["kernel_ms_per_evaluation"] = {
+            k: round(float(v), 4) for k, v in sim.kernel_times().items()
+            if k in ("mlp_seed", "mlp_train", "density", "predict", "nblist", "force")}
         sim.close()
     for k, (a, b) in {"h": (fields["mlp"][0], fields["plain_fp64"][0]),
                       "rho": (fields["mlp"][1], fields["plain_fp64"][1])}.items():
@@ -211,10 +219,15 @@ def run_sweep(nsim, steps):
 
 
 def field_errors(nside):
-    """mixed vs FP64 GPU on one derivs of the same state (norm-relative L-inf)"""
+    """mixed vs FP64 GPU on one derivs of the same state (norm-relative L-inf),
+    the box's uniform B perturbed by a random tangled component"""
     import ndspmhd_amd as nd
     from ndspmhd_amd.setups import periodic_mhd_box
     o, st = periodic_mhd_box(nside)
+    # a tangled field on top of the uniform one, so that divB and the
+    # Lorentz force are not zero
+    rng = np.random.default_rng(17)
+    st["Bevol"] = st["Bevol"] + 0.05 * rng.standard_normal(st["Bevol"].shape)
     o = o.replace(maxdensits=0)
     a = nd.Simulation(o, st, precision=0)
     b = nd.Simulation(o, st, precision=1)
@@ -330,7 +343,7 @@ def main():
                          "hbm_gbs": b / t / 1e9, "hbm_frac": b / t / 1e9 / HBM_PEAK_GBS,
                          "fp32_tflops": f / t / 1e12, "fp32_frac": f / t / 1e12 / FP32_PEAK_TFLOPS,
                          "roof_frac": t_roof / t,
-                         "traffic": traffic_from_profile({"density": "k_nr_pass"}.get(k, "k_" + k))}
+                         "traffic": traffic_from_profile({"density": "density_evaluation"}.get(k, "k_" + k))}
     dom = max(per_kernel, key=lambda k: per_kernel[k]["ms"]) if per_kernel else "force"
     alg_bytes = per_kernel[dom]["alg_bytes"]
     dom_ms = per_kernel[dom]["ms"]
@@ -353,6 +366,7 @@ def main():
                                "gamma 5/3, v_rms Mach 1 P(k)~k^-4, B=(0.1,0,0)" % (args.nside, n),
                    "particles": n, "steps": args.steps},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "limited_by": "VALU issue and gather latency (SQ counters), not HBM bandwidth",
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": per_kernel[dom]["traffic"],
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": dom_ms,

@@ -274,22 +274,29 @@ __global__ __launch_bounds__(256) void k_mlp_seed_mfma(const uint16_t* __restric
   mlp_lane_weights(wbf, w);
   const int lane = threadIdx.x & 63, i = lane & 31, hb = lane >> 5;
   const long long base = ((long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63));
+  const float dtf = (float)f.dt, nexpf = (float)f.nexp, inv_nexp = (float)(1.0 / f.nexp);
+  const float dndimf = (float)f.dndim, inv_hfact = (float)(1.0 / f.hfact);
   for (int r = 0; r < 2; r++) {
     const long long p = base + 32 * r + i;
     const bool ok = p < f.n;
     float x[4] = {0.f, 0.f, 0.f, 0.f};
     double hpred = 0.0;
+    // features in FP32 (network inputs; the same arithmetic forms the
+    // training rows), the half-wave taking features 0-3 or 4-5
     if (ok) {
-      const double hin = f.hin[p], rho = f.rho[p];
       hpred = f.h[p];
+      const float hin = (float)f.hin[p], rho = (float)f.rho[p];
       if (hb == 0) {
-        x[0] = (float)(100.0 * f.dt * f.dhdtin[p] / hin);
-        x[1] = rho > 0.0 ? (float)(100.0 * f.dt * f.drhodt[p] / rho) : 0.0f;
+        x[0] = 100.0f * dtf * (float)f.dhdtin[p] * __builtin_amdgcn_rcpf(hin);
+        x[1] = rho > 0.0f ? 100.0f * dtf * (float)f.drhodt[p] * __builtin_amdgcn_rcpf(rho) : 0.0f;
         x[2] = (float)(1000.0 * f.corr[p]);
-        x[3] = (float)((f.numneigh[p] - f.nexp) / f.nexp);
+        x[3] = ((float)f.numneigh[p] - nexpf) * inv_nexp;
       } else {
-        x[0] = (float)(f.nits[p] - 1.5);
-        x[1] = rho > 0.0 ? (float)(1000.0 * (hin * pow(rho / f.m[p], f.dndim) / f.hfact - 1.0)) : 0.0f;
+        x[0] = (float)f.nits[p] - 1.5f;
+        x[1] = rho > 0.0f
+                   ? 1000.0f * (hin * exp2f(dndimf * __log2f(rho * __builtin_amdgcn_rcpf((float)f.m[p]))) * inv_hfact -
+                                1.0f)
+                   : 0.0f;
       }
     }
     const float out = mlp_mfma_block(w, x);
@@ -307,7 +314,7 @@ __global__ __launch_bounds__(256) void k_mlp_seed_mfma(const uint16_t* __restric
     if (f.apply && hb == 0 && !(f.itype && f.itype[p] == 1)) {
       float yy = out / MLP_TSCALE;
       yy = fminf(fmaxf(yy, -0.1f), 0.1f);
-      if (isfinite(yy) && yy != 0.0f && hpred > 0.0) f.h[p] = hpred * exp((double)yy);
+      if (isfinite(yy) && yy != 0.0f && hpred > 0.0) f.h[p] = hpred * (double)__expf(yy);
     }
   }
 }

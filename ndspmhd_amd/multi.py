@@ -9,9 +9,9 @@ together (ndsp_batch_step), each writing its own .ev file and dumps.
     python -m ndspmhd_amd.multi --write-shock-sweep shk 7     (multi_shock)
 
 Shock tubes are set up as src/setup_shock1D_mhd.f90 does in 1D (particle
-placement, smoothed states, equal masses) with reflecting walls at
-x = -0.5, 0.5 where the reference holds six boundary particles fixed
-(ibound 1, not implemented by this library).
+placement, smoothed states, equal masses), with the first and last six
+particles held fixed as the reference's boundary particles (ibound 1,
+src/setup_shock1D_mhd.f90:47-48).
 """
 from __future__ import annotations
 
