@@ -4,9 +4,9 @@
 // Pipeline of one derivative evaluation (src/derivs.f90:27):
 //   k_boundary      periodic wrap / wall reflection (src/boundaryND.f90:66-160)
 //   k_binstats      max h and bounding box (src/ghostND_mhd.f90:77, linkND.f90:78-90)
-//   k_cellid, scan, k_scatter, k_cellsort
-//                   counting sort of particles by linear cell key (x fastest),
-//                   stable by slot: replaces the head/next link lists of
+//   k_cellid, scan, k_scatter, k_cellsort_wave
+//                   counting sort of particles by Morton cell key, stable
+//                   by slot: replaces the head/next link lists of
 //                   set_linklist (src/linkND.f90:45-161)
 //   k_pack_density  sorted neighbour view: FP64 (x,y,z,1/h) + payload
 //   k_nr_pass       h-rho Newton-Raphson passes, persistent once a pass is partial
@@ -169,8 +169,8 @@ __global__ __launch_bounds__(BS) void k_binstats_final(DScal* sc, const double* 
 // side >= radkern*hmax/cell_sub (src/linkND.f90:72 uses radkern*hhmax);
 // walls tile their dimension exactly; cells are widened when the Morton key
 // space would exceed 2^cap_bits.  Also decides whether neighbour lists can
-// be used (2*radkern*hmax*(1+skin) below every periodic length, so the
-// minimum image of a neighbour is unique).
+// be used (radkern*hmax*(1+skin) below every periodic length, so every image
+// within a list radius lies in the image cells one period away).
 __device__ __forceinline__ void b_grid_setup(DScal* sc, Grid* gd, Grid box, Phys p, int cap_bits, VL vl) {
   if (threadIdx.x | vl.b) return;
   Grid g = box;
@@ -217,7 +217,7 @@ __device__ __forceinline__ void b_grid_setup(DScal* sc, Grid* gd, Grid box, Phys
   g.ncell = 1LL << (p.ndim * mb);
   int on = p.use_list;
   for (int d = 0; d < p.ndim; d++)
-    if (g.bound[d] == 3 && !(2.0 * rmax * (1.0 + p.skin) * 1.001 < g.xmax[d] - g.xmin[d])) on = 0;
+    if (g.bound[d] == 3 && !(rmax * (1.0 + p.skin) * 1.001 < g.xmax[d] - g.xmin[d])) on = 0;
   sc->list_on = on;
   sc->list_overflow = 0;
   *gd = g;
@@ -367,37 +367,34 @@ __global__ void k_scatter(KArgs a) {
 }
 
 // make the order inside each cell ascending in slot index (deterministic,
-// and a stable sort by cell key overall)
-__device__ __forceinline__ void b_cellsort(KArgs a, const Grid* __restrict__ gp, VL vl) {
-  long long c = (long long)vl.b * blockDim.x + threadIdx.x;
-  if (c >= gp->ncell) return;
-  int b = a.cstart[c], e = a.cstart[c + 1];
-  for (int k = b + 1; k < e; k++) {
-    int v = a.sidx[k];
-    int j = k - 1;
-    while (j >= b && a.sidx[j] > v) {
-      a.sidx[j + 1] = a.sidx[j];
-      j--;
-    }
-    a.sidx[j + 1] = v;
-  }
-}
-__global__ void k_cellsort(KArgs a, const Grid* __restrict__ gp) {
-  b_cellsort(a, gp, VL{(int)blockIdx.x, (int)gridDim.x});
-}
-// the same order, with the whole wave sorting each cell of its 64 cells in
-// turn: a cell of at most 256 particles is ranked in registers (each slot
-// index counts the smaller ones of its cell), a larger one is sorted by the
-// first lane as above
-__global__ __launch_bounds__(BS) void k_cellsort_wave(KArgs a, const Grid* __restrict__ gp) {
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// and a stable sort by cell key overall), the whole wave sorting each cell
+// of its 64 cells in turn: a cell of at most 256 particles is ranked in
+// registers (each slot index counts the smaller ones of its cell), a larger
+// one is insertion-sorted by the first lane.  Cells of at most CS_LANE
+// particles (most of them) are ranked first, each by its own lane, all at once.
+#define CS_LANE 16
+__device__ __forceinline__ void b_cellsort_wave(KArgs a, const Grid* __restrict__ gp, VL vl) {
+  const long long c = (long long)vl.b * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   int b = 0, m = 0;
   if (c < gp->ncell) {
     b = a.cstart[c];
     m = a.cstart[c + 1] - b;
   }
-  unsigned long long todo = __ballot(m > 1);
+  if (m > 1 && m <= CS_LANE) {
+    // every entry is read before any is written back
+    int v[CS_LANE];
+#pragma unroll
+    for (int q = 0; q < CS_LANE; q++) v[q] = q < m ? a.sidx[b + q] : INT_MAX;
+#pragma unroll
+    for (int q = 0; q < CS_LANE; q++) {
+      int r = 0;
+#pragma unroll
+      for (int t = 0; t < CS_LANE; t++) r += v[t] < v[q] ? 1 : 0;
+      if (q < m) a.sidx[b + r] = v[q];
+    }
+  }
+  unsigned long long todo = __ballot(m > CS_LANE);
   while (todo) {
     const int src = __ffsll((long long)todo) - 1;
     todo &= todo - 1ull;
@@ -436,6 +433,9 @@ __global__ __launch_bounds__(BS) void k_cellsort_wave(KArgs a, const Grid* __res
       }
     }
   }
+}
+__global__ __launch_bounds__(BS) void k_cellsort_wave(KArgs a, const Grid* __restrict__ gp) {
+  b_cellsort_wave(a, gp, VL{(int)blockIdx.x, (int)gridDim.x});
 }
 
 // ===========================================================================
@@ -647,27 +647,32 @@ __device__ __forceinline__ void for_each_candidate_rec(const KArgs& a, const Gri
 
 // Neighbour lists (periodic or open domains).  Entry m of sorted particle k
 // sits at nbl[k*maxnb + m] = (sorted index << 7) | image code.  Entries
-// [0, nbc) are the gather list G(k): every j with r_kj < radkern*h_k*(1+skin),
-// in the candidate order of the cell scan; then nbx transposed extras: every
-// j with k in G(j) but j not in G(k).  Together they hold every pair with
-// r < radkern*max(h_k, h_j) while no h has grown past hcap = h*(1+skin).
-// Membership uses the minimum-image separation of the raw positions, which is
-// exactly antisymmetric in FP64 (fl(a-b) = -fl(b-a), and so are the +-L
-// shifts), so "k in G(j)" is decided identically from k's and from j's side
-// and no pair is lost or doubled.  Lists need 2*radkern*hmax*(1+skin) < L along
-// periodic dimensions (checked by the host before every build), so the only
-// copy of j within the list radius is its minimum image.
+// [0, nbc) are the gather list G(k): every image of a particle j with
+// r_kj < radkern*h_k*(1+skin), in the candidate order of the cell scan; then
+// nbx transposed extras: every image of j whose opposite image of k is in
+// G(j) while it is not in G(k) itself.  Together they hold every pair (and
+// pair of periodic images) with r < radkern*max(h_k, h_j) while no h has
+// grown past hcap = h*(1+skin).  Membership uses the separation
+// fl(x_k - x_j) + s L per dimension, the shift s = +1, -1 or 0 given by the
+// image code, summed without contraction: it is exactly antisymmetric in
+// FP64 (fl(a - b) = -fl(b - a), and the shifts of two opposite images are
+// exact negatives), so "k in G(j)" is decided identically from k's and from
+// j's side and no pair is lost or doubled.  Lists need
+// radkern*hmax*(1+skin) < L along periodic dimensions (checked on the device
+// at every binning), so every image within a list radius lies in the image
+// cells one period away; where 2*radkern*hmax*(1+skin) >= L a list can hold
+// two images of one neighbour, or an image of the particle itself, as the
+// reference's ghost copies interact (src/ghostND_mhd.f90:180-200).
 template <int NDIM>
-__device__ __forceinline__ double min_image_r2(const Grid& g, const double* xa, const double* xb) {
+__device__ __forceinline__ double image_r2(const Grid& g, const double* xa, const double* xb, unsigned code) {
 #pragma clang fp contract(off)
   double r2 = 0.0;
   for (int d = 0; d < NDIM; d++) {
     double dx = xa[d] - xb[d];
-    if (g.bound[d] == 3) {
-      const double L = g.xmax[d] - g.xmin[d];
-      if (dx > 0.5 * L) dx = dx - L;
-      else if (dx < -0.5 * L) dx = dx + L;
-    }
+    const unsigned kd = (code >> (2 * d)) & 3u;
+    // kind 1: the image left of xmin (at x_j - L); kind 2: right of xmax
+    if (kd == 1u) dx = dx + (g.xmax[d] - g.xmin[d]);
+    else if (kd == 2u) dx = dx - (g.xmax[d] - g.xmin[d]);
     r2 = r2 + dx * dx;
   }
   return r2;
@@ -698,7 +703,7 @@ __device__ __forceinline__ int wave_max_i(int v) {
 // particle for a prefilter (relative margin 1e-4 on r^2, far above the FP32
 // rounding of the separations within a compact group).  Each lane marks the
 // candidates that pass the prefilter, then applies the exact FP64
-// minimum-image test to those.  A pair k-j with j in G(k) and k not in G(j) is
+// image test to those.  A pair k-j with j in G(k) and k not in G(j) is
 // appended to the extras of j, which fill j's row from its end
 // (k_nblist_sort moves them into place).
 #define NBL_CELLS 256
@@ -780,14 +785,15 @@ __device__ __forceinline__ void b_nblist(KArgs a, Phys p, const Grid* __restrict
       mask &= mask - 1ull;
       const double4 c = sraw[t];
       const double xr[3] = {c.x, c.y, c.z};
-      const double rr = min_image_r2<NDIM>(g, xi, xr);
-      if (!(rr * hi21 < lim2)) continue;
       const unsigned e = se[t];
+      const double rr = image_r2<NDIM>(g, xi, xr, e & 63u);
+      if (!(rr * hi21 < lim2)) continue;
       if (cnt < a.maxnb) L[cnt] = e;
       else over = true;
       cnt++;
       const int kk = (int)(e >> 7);
-      // same operands as kk's own test of k (rr is symmetric)
+      // same operands as kk's own test of the opposite image of k (rr is
+      // symmetric); an image of k itself has its opposite in G(k) already
       if (kk != k && !(rr * (c.w * c.w) < lim2)) {
         const int slot = atomicAdd(&a.nbx[kk], 1);
         if (slot < a.maxnb)
@@ -2583,7 +2589,7 @@ NDSP_MK(km_scan_local) { NDSP_MEMBER b_scan_local(M.a.cfill, M.a.cstart, M.bsum,
 NDSP_MK(km_scan_bsum) { NDSP_MEMBER b_scan_bsum(M.bsum, M.gdev, nullptr, vl); }
 NDSP_MK(km_scan_add) { NDSP_MEMBER b_scan_add(M.a.cstart, M.bsum, M.gdev, vl); }
 NDSP_MK(km_scatter) { NDSP_MEMBER b_scatter(M.a, vl); }
-NDSP_MK(km_cellsort) { NDSP_MEMBER b_cellsort(M.a, M.gdev, vl); }
+NDSP_MK(km_cellsort) { NDSP_MEMBER b_cellsort_wave(M.a, M.gdev, vl); }
 NDSP_MK(km_fill_nbx) { NDSP_MEMBER b_fill_i(M.a.nbx, M.a.n, 0, vl); }
 NDSP_MK(km_nblist_sort) { NDSP_MEMBER b_nblist_sort(M.a, vl); }
 NDSP_MK(km_init_density) { NDSP_MEMBER b_init_density_scalars(M.a.sc, M.bconstmax, M.a.n, vl); }
@@ -2713,6 +2719,7 @@ struct ndsp_ctx {
   bool reflect = false;     // some wall reflects
   Grid* gdev = nullptr;     // cell grid of the last binning (device)
   int cap_bits = 0;         // Morton key-space capacity of cstart/cfill
+  int maxnb_limit = 0;      // largest list capacity the lists may grow to
   bool capturing = false;   // a step is being recorded into a graph
   bool use_graph = false;   // ndsp_step replays recorded graphs
   bool graph_valid = false;
@@ -2850,6 +2857,28 @@ static int size_key_space(ndsp_ctx* c, const DScal& hs, int extra_bits) {
   HIPCHK(hipMalloc(&c->a.cfill, sizeof(int) * cap));
   HIPCHK(hipMalloc(&c->bsum, sizeof(int) * (cap / (BS * 4) + 2)));
   c->cap_bits = bits;
+  c->graph_valid = false;
+  return 0;
+}
+
+// List capacity after a binning in which some rows overflowed (those
+// particles scanned cells, the slowest part of their launch): 1.5 times the
+// present one, up to four times the initial capacity; the old lists are
+// dropped (the next binning rebuilds them) and recorded steps are recorded
+// again.  A failed allocation keeps the present capacity for good.
+static int grow_lists(ndsp_ctx* c) {
+  if (!c->p.use_list || c->a.maxnb >= c->maxnb_limit) return 0;
+  const int m = std::min(c->maxnb_limit, (c->a.maxnb * 3 / 2 + 31) / 32 * 32);
+  unsigned* nbl = nullptr;
+  if (hipMalloc(&nbl, sizeof(unsigned) * (size_t)c->n * (size_t)m) != hipSuccess) {
+    (void)hipGetLastError();
+    c->maxnb_limit = c->a.maxnb;
+    return 0;
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipFree(c->a.nbl));
+  c->a.nbl = nbl;
+  c->a.maxnb = m;
   c->graph_valid = false;
   return 0;
 }
@@ -3261,6 +3290,7 @@ int ndsp_create(const int32_t* iopt, const double* dopt, int64_t npart, int32_t 
     const double r = p.radkern * p.hfact * (1.0 + p.skin);
     const double vol = p.ndim == 3 ? 4.18879020478639 * r * r * r : p.ndim == 2 ? M_PI * r * r : 2.0 * r;
     a.maxnb = ((int)(1.8 * vol + 8.0) + 31) / 32 * 32;
+    c->maxnb_limit = 4 * a.maxnb;
   }
   if (p.use_list) {
     ok &= hipMalloc(&a.nbl, sizeof(unsigned) * n * (size_t)a.maxnb) == hipSuccess;
@@ -3627,6 +3657,8 @@ int ndsp_step(ndsp_ctx* c, ndsp_stats* stats, ndsp_dt* dtout) {
   c->nsteps++;
   if (hs.grid_clamped && c->cap_bits < 28)
     if (int e = size_key_space(c, hs, 2)) return e;
+  if (hs.list_overflow)
+    if (int e = grow_lists(c)) return e;
   if (c->mlp.enabled) c->mlp.steps_seen++;
   fill_outputs(c, hs, stats, dtout);
   return check_err(c, hs);
@@ -3661,6 +3693,7 @@ struct ndsp_batch {
   DScal* hs = nullptr;               // pinned host copies of every member's scalars
   DScal* dsc = nullptr;              // the members' scalars gathered on the device
   std::vector<int> caps;             // members' key-space sizes at recording
+  std::vector<int> nbcaps;           // members' list capacities at recording
   std::vector<BGroup> groups;
   int prec = 0;
   int tb = 256;                      // workgroup size of the pair kernels in batch launches
@@ -3944,7 +3977,7 @@ int ndsp_batch_step(ndsp_batch* b, ndsp_stats* stats, ndsp_dt* dt) {
     if (int e = ensure_key_space(c)) return e;
     due[s] = c->grid_valid && c->reorder_every > 0 && (c->nsteps % c->reorder_every == 0 || c->reorder_next);
     if (due[s]) c->reorder_next = false;
-    if (b->caps.size() != (size_t)ns || b->caps[s] != c->cap_bits) b->valid = false;
+    if (b->caps.size() != (size_t)ns || b->caps[s] != c->cap_bits || b->nbcaps[s] != c->a.maxnb) b->valid = false;
   }
   for (int s = 0; s < ns; s++) HIPCHK(hipStreamSynchronize(b->sims[s]->st));
   if (!b->valid) {
@@ -3952,7 +3985,11 @@ int ndsp_batch_step(ndsp_batch* b, ndsp_stats* stats, ndsp_dt* dt) {
     if (int e = batch_capture(b, b->ga, batch_part_a)) return e;
     if (int e = batch_capture(b, b->gb, batch_part_b)) return e;
     b->caps.resize(ns);
-    for (int s = 0; s < ns; s++) b->caps[s] = b->sims[s]->cap_bits;
+    b->nbcaps.resize(ns);
+    for (int s = 0; s < ns; s++) {
+      b->caps[s] = b->sims[s]->cap_bits;
+      b->nbcaps[s] = b->sims[s]->a.maxnb;
+    }
     b->valid = true;
   }
   if (std::find(due.begin(), due.end(), 1) != due.end())
@@ -3977,6 +4014,8 @@ int ndsp_batch_step(ndsp_batch* b, ndsp_stats* stats, ndsp_dt* dt) {
     c->nsteps++;
     if (hs.grid_clamped && c->cap_bits < 28)
       if (int e = size_key_space(c, hs, 2)) return e;
+    if (hs.list_overflow)
+      if (int e = grow_lists(c)) return e;
     fill_outputs(c, hs, stats ? stats + s : nullptr, dt ? dt + s : nullptr);
     const int e = check_err(c, hs);
     if (e && !first_err) first_err = e;

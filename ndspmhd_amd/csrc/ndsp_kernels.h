@@ -72,7 +72,7 @@ struct DScal {
   int relink;
   int list_invalid;    // some h grew past its list's margin: force pass scans cells
   int list_overflow;   // particles whose list did not fit (they scan cells)
-  int list_on;         // lists built by the last binning (minimum image unique)
+  int list_on;         // lists built by the last binning (images one period away)
   int grid_clamped;    // cells were widened to fit the key-space capacity
   double dt_cur;       // dt of the step in progress / the next step
   // h-rho pass control, advanced on the device by k_pass_ctl

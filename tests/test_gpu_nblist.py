@@ -106,3 +106,29 @@ def test_lists_fall_back_where_they_overflow(prec, tol):
     assert "nblist" in n0
     assert np.array_equal(c0[0], c1[0])
     _close(s0, s1, tol)
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-12), (0, 1e-6)])
+def test_lists_holding_both_images_of_a_neighbour(prec, tol):
+    """2D tube (fixed x walls) whose periodic y extent, 10 lattice rows, lies
+    between one and two list radii of its largest h: a list then holds the
+    images of a neighbour on both sides, and images of its own particle, as
+    the reference's ghost copies interact; the results equal the scan's"""
+    from ndspmhd_amd.setups import shock_tube_2d
+    o, st = shock_tube_2d(800, hfact=1.2, width_rows=10)
+    Ly = o.xmax[1] - o.xmin[1]
+    rlist = 2.0 * st["hh"].max() * 1.04  # radkern hmax (1 + skin), cubic spline
+    print("2D tube, Ly %.4g, list radius %.4g, prec %d" % (Ly, rlist, prec), flush=True)
+    assert rlist * 1.001 < Ly < 2.0 * rlist
+    import ndspmhd_amd as nd
+    probe = nd.Simulation(o, st, precision=prec)
+    probe.derivs()
+    L = probe.get("nblist_len")
+    # lists are built (a few rows at the contact, where h jumps, overflow
+    # their capacity and scan cells)
+    print("list fallbacks", int((L < 0).sum()), "of", L.size, flush=True)
+    assert (L >= 0).mean() > 0.9
+    (s0, n0, c0), (s1, n1, c1) = _pair(o, st, prec, 4)
+    assert "nblist" in n0
+    assert np.array_equal(c0[0], c1[0]) and np.array_equal(c0[1], c1[1])
+    _close(s0, s1, tol)
