@@ -2553,7 +2553,7 @@ struct MArgs {
   int reflect;
   int nbp;        // binning-statistics blocks
   int nb_nr, nb_force, nb_256;  // blocks of the h-rho, force and elementwise launches
-  double* scratch_d;  // one field of n doubles / ints for reorders
+  double* scratch_d;  // REORDER_W fields of n doubles (or the five int fields) for reorders
   int* scratch_i;
 };
 
@@ -2607,23 +2607,33 @@ NDSP_MK(km_gather_scalars) {
   int* dst = (int*)M.out;
   for (int w = threadIdx.x; w < (int)(sizeof(DScal) / sizeof(int)); w += blockDim.x) dst[w] = src[w];
 }
-// one field of the periodic reorder (slot k <- sorted position k) of the
-// members flagged in due: phase 0 gathers it into the member's scratch,
-// phase 1 copies it back; fields past F_NPERSIST are the ids and counters
+// a block of fields of the periodic reorder (slot k <- sorted position k) of
+// the members flagged in due: phase 0 gathers fields f0 .. f0+REORDER_W-1
+// (up to F_NPERSIST) into the member's scratch, phase 1 copies them back;
+// f0 = F_NPERSIST takes the ids and counters (five int fields, held in the
+// same scratch)
 __global__ void km_reorder(const MArgs* __restrict__ ma, const int* __restrict__ off, int nm,
-                           const int* __restrict__ due, int fi, int phase) {
+                           const int* __restrict__ due, int f0, int phase) {
   NDSP_MEMBER
   if (!due[m_]) return;
   const int k = vl.b * blockDim.x + threadIdx.x;
-  if (k >= M.a.n) return;
-  if (fi < F_NPERSIST) {
-    if (phase == 0) M.scratch_d[k] = M.a.f[fi][M.a.sidx[k]];
-    else M.a.f[fi][k] = M.scratch_d[k];
+  const int n = M.a.n;
+  if (k >= n) return;
+  if (f0 < F_NPERSIST) {
+    const int nf = min(REORDER_W, F_NPERSIST - f0);
+    const int src = phase == 0 ? M.a.sidx[k] : 0;
+    for (int q = 0; q < nf; q++) {
+      if (phase == 0) M.scratch_d[(size_t)q * n + k] = M.a.f[f0 + q][src];
+      else M.a.f[f0 + q][k] = M.scratch_d[(size_t)q * n + k];
+    }
   } else {
     int* const iv[5] = {M.a.uid, M.a.numneigh, M.a.nits, M.a.itype, M.a.ireal};
-    int* v = iv[fi - F_NPERSIST];
-    if (phase == 0) M.scratch_i[k] = v[M.a.sidx[k]];
-    else v[k] = M.scratch_i[k];
+    int* si = (int*)M.scratch_d;
+    const int src = phase == 0 ? M.a.sidx[k] : 0;
+    for (int q = 0; q < 5; q++) {
+      if (phase == 0) si[(size_t)q * n + k] = iv[q][src];
+      else iv[q][k] = si[(size_t)q * n + k];
+    }
   }
 }
 // reductions: which[0] = h-rho pass (skipped once finished), 1 = force, 2 = finalize
@@ -3694,6 +3704,8 @@ struct ndsp_batch {
   DScal* dsc = nullptr;              // the members' scalars gathered on the device
   std::vector<int> caps;             // members' key-space sizes at recording
   std::vector<int> nbcaps;           // members' list capacities at recording
+  std::vector<hipStream_t> gst;      // streams of the groups after the first (recording only)
+  std::vector<hipEvent_t> gev;       // fork event, then one join event per further group
   std::vector<BGroup> groups;
   int prec = 0;
   int tb = 256;                      // workgroup size of the pair kernels in batch launches
@@ -3731,6 +3743,16 @@ static int batch_build(ndsp_batch* b) {
     }
     g->mem.push_back(s);
     g->fixed |= c->p.fixed_on;
+  }
+  while (b->gst.size() + 1 < b->groups.size()) {
+    hipStream_t st = nullptr;
+    HIPCHK(hipStreamCreate(&st));
+    b->gst.push_back(st);
+  }
+  while (b->gev.size() < b->groups.size()) {
+    hipEvent_t ev = nullptr;
+    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    b->gev.push_back(ev);
   }
   for (auto& g : b->groups) {
     const int nm = (int)g.mem.size();
@@ -3780,15 +3802,37 @@ static int batch_build(ndsp_batch* b) {
   return 0;
 }
 
-// launch of a batch kernel over shape k of group g
+// launch of a batch kernel over shape k of group g, on stream bst
 #define BLAUNCH(kern, g, k, threads, ...)                                                                   \
-  hipLaunchKernelGGL(kern, dim3((g).tot[k]), dim3(threads), 0, b->st, (const MArgs*)(g).args,               \
+  hipLaunchKernelGGL(kern, dim3((g).tot[k]), dim3(threads), 0, bst, (const MArgs*)(g).args,                 \
                      (const int*)(g).off[k], (int)(g).mem.size(), ##__VA_ARGS__)
+
+// In a recorded half step every group after the first runs on a stream of
+// its own, forked from the batch stream and joined back before the scalars
+// are copied out: the groups are independent, so the graph holds one branch
+// per group and their phases overlap on the device.
+static hipStream_t batch_group_stream(ndsp_batch* b, size_t gi) { return gi == 0 ? b->st : b->gst[gi - 1]; }
+static int batch_fork(ndsp_batch* b) {
+  if (b->groups.size() < 2) return 0;
+  HIPCHK(hipEventRecord(b->gev[0], b->st));
+  for (size_t gi = 1; gi < b->groups.size(); gi++) HIPCHK(hipStreamWaitEvent(b->gst[gi - 1], b->gev[0], 0));
+  return 0;
+}
+static int batch_join(ndsp_batch* b) {
+  for (size_t gi = 1; gi < b->groups.size(); gi++) {
+    HIPCHK(hipEventRecord(b->gev[gi], b->gst[gi - 1]));
+    HIPCHK(hipStreamWaitEvent(b->st, b->gev[gi], 0));
+  }
+  return 0;
+}
 
 // first half of a step for every member (as step_part_a), then the scalars
 static int batch_part_a(ndsp_batch* b) {
   const bool fp64 = b->prec == NDSP_PREC_FP64;
-  for (auto& g : b->groups) {
+  if (int e = batch_fork(b)) return e;
+  for (size_t gi = 0; gi < b->groups.size(); gi++) {
+    auto& g = b->groups[gi];
+    const hipStream_t bst = batch_group_stream(b, gi);
     BLAUNCH(km_predict, g, BGroup::S_256, BS);
     if (g.fixed) BLAUNCH(km_predict_fixed, g, BGroup::S_256, BS);
     BLAUNCH(km_err_reset, g, BGroup::S_ONE, 64);
@@ -3840,6 +3884,7 @@ static int batch_part_a(ndsp_batch* b) {
     }
     BLAUNCH(km_gather_scalars, g, BGroup::S_ONE, 64);
   }
+  if (int e = batch_join(b)) return e;
   HIPCHK(hipMemcpyAsync(b->hs, b->dsc, sizeof(DScal) * b->sims.size(), hipMemcpyDeviceToHost, b->st));
   HIPCHK(hipGetLastError());
   return 0;
@@ -3848,7 +3893,10 @@ static int batch_part_a(ndsp_batch* b) {
 // second half of a step for every member (as step_part_b), then the scalars
 static int batch_part_b(ndsp_batch* b) {
   const bool fp64 = b->prec == NDSP_PREC_FP64;
-  for (auto& g : b->groups) {
+  if (int e = batch_fork(b)) return e;
+  for (size_t gi = 0; gi < b->groups.size(); gi++) {
+    auto& g = b->groups[gi];
+    const hipStream_t bst = batch_group_stream(b, gi);
     if (g.fixed) BLAUNCH(km_fixed_copy, g, BGroup::S_256, BS);
     if (fp64) BLAUNCH(km_pack_force<double>, g, BGroup::S_256, BS);
     else BLAUNCH(km_pack_force<float>, g, BGroup::S_256, BS);
@@ -3878,6 +3926,7 @@ static int batch_part_b(ndsp_batch* b) {
     BLAUNCH(km_step_end, g, BGroup::S_ONE, 64);
     BLAUNCH(km_gather_scalars, g, BGroup::S_ONE, 64);
   }
+  if (int e = batch_join(b)) return e;
   HIPCHK(hipMemcpyAsync(b->hs, b->dsc, sizeof(DScal) * b->sims.size(), hipMemcpyDeviceToHost, b->st));
   HIPCHK(hipGetLastError());
   return 0;
@@ -3886,6 +3935,7 @@ static int batch_part_b(ndsp_batch* b) {
 // the periodic reorder of the flagged members (as reorder() does for one
 // context), each field one launch over all members of a group
 static int batch_reorder(ndsp_batch* b, const std::vector<int>& due) {
+  const hipStream_t bst = b->st;
   for (auto& g : b->groups) {
     const int nm = (int)g.mem.size();
     std::vector<int> d(nm);
@@ -3893,9 +3943,13 @@ static int batch_reorder(ndsp_batch* b, const std::vector<int>& due) {
     for (int q = 0; q < nm; q++) any |= (d[q] = due[g.mem[q]]) != 0;
     if (!any) continue;
     HIPCHK(hipMemcpyAsync(g.due, d.data(), sizeof(int) * nm, hipMemcpyHostToDevice, b->st));
-    for (int fi = 0; fi < F_NPERSIST + 5; fi++)
+    // blocks of REORDER_W double fields, then the int fields (f0 = F_NPERSIST)
+    for (int f0 = 0; f0 < F_NPERSIST + REORDER_W; f0 += REORDER_W) {
+      const int b0 = std::min(f0, (int)F_NPERSIST);
       for (int phase = 0; phase < 2; phase++)
-        BLAUNCH(km_reorder, g, BGroup::S_256, BS, (const int*)g.due, fi, phase);
+        BLAUNCH(km_reorder, g, BGroup::S_256, BS, (const int*)g.due, b0, phase);
+      if (b0 == F_NPERSIST) break;
+    }
     BLAUNCH(km_slot_of, g, BGroup::S_256, BS);
     // the host vector d must outlive the copy
     HIPCHK(hipStreamSynchronize(b->st));
@@ -3953,6 +4007,11 @@ int ndsp_batch_destroy(ndsp_batch* b) {
   if (b->ga) hipGraphExecDestroy(b->ga);
   if (b->gb) hipGraphExecDestroy(b->gb);
   batch_free_groups(b);
+  for (hipStream_t st : b->gst) {
+    hipStreamSynchronize(st);
+    hipStreamDestroy(st);
+  }
+  for (hipEvent_t ev : b->gev) hipEventDestroy(ev);
   if (b->hs) hipHostFree(b->hs);
   if (b->dsc) hipFree(b->dsc);
   if (b->st) hipStreamDestroy(b->st);
