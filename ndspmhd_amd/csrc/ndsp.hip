@@ -167,10 +167,11 @@ __global__ __launch_bounds__(BS) void k_binstats_final(DScal* sc, const double* 
 
 // Cell grid of this binning, on the device (no host round trip): cells of
 // side >= radkern*hmax/cell_sub (src/linkND.f90:72 uses radkern*hhmax);
-// walls tile their dimension exactly; cells are widened when the Morton key
-// space would exceed 2^cap_bits.  Also decides whether neighbour lists can
-// be used (radkern*hmax*(1+skin) below every periodic length, so every image
-// within a list radius lies in the image cells one period away).
+// walls tile their dimension exactly; the dimensions with the most key bits
+// give up bits (their cells widen) while the Morton key space would exceed
+// 2^cap_bits.  Also decides whether neighbour lists can be used
+// (radkern*hmax*(1+skin) below every periodic length, so every image within
+// a list radius lies in the image cells one period away).
 __device__ __forceinline__ void b_grid_setup(DScal* sc, Grid* gd, Grid box, Phys p, int cap_bits, VL vl) {
   if (threadIdx.x | vl.b) return;
   Grid g = box;
@@ -179,42 +180,73 @@ __device__ __forceinline__ void b_grid_setup(DScal* sc, Grid* gd, Grid box, Phys
   sc->grid_clamped = 0;
   if (!(rmax > 0.0) || !(rmax < DBL_MAX)) sc->err = NDSP_ERR_CELLS;
   double cs = rmax / p.cell_sub;
-  const int mbmax = cap_bits / p.ndim;
+  double ncd[3] = {1.0, 1.0, 1.0};
+  for (int d = 0; d < p.ndim; d++) {
+    if (g.bound[d] >= 2) {
+      const double L = g.xmax[d] - g.xmin[d];
+      if (!(L >= rmax)) sc->err = NDSP_ERR_CELLS;
+      ncd[d] = floor(L / cs);
+      if (!(ncd[d] >= 1.0)) ncd[d] = 1.0;
+    } else {
+      ncd[d] = floor((sc->bbox_hi[d] - sc->bbox_lo[d]) / cs) + 1.0;
+      if (!(ncd[d] >= 1.0)) ncd[d] = 1.0;
+    }
+    ncd[d] = fmin(ncd[d], (double)(1 << 28));
+  }
+  int kb[3] = {0, 0, 0}, tot = 0;
+  for (int d = 0; d < p.ndim; d++) {
+    while ((double)(1 << kb[d]) < ncd[d]) kb[d]++;
+    tot += kb[d];
+  }
+  bool clamped[3] = {false, false, false};
+  while (tot > cap_bits) {
+    int dm = 0;
+    for (int d = 1; d < p.ndim; d++)
+      if (kb[d] > kb[dm]) dm = d;
+    kb[dm]--;
+    tot--;
+    clamped[dm] = true;
+    sc->grid_clamped = 1;
+  }
   for (int d = 0; d < 3; d++) {
+    g.kb[d] = kb[d];
     if (d >= p.ndim) {
       g.nc[d] = 1; g.lo[d] = 0.0; g.w[d] = 1.0; g.inv_w[d] = 1.0;
       continue;
     }
-    double ncd;
+    if (clamped[d]) ncd[d] = fmin(ncd[d], (double)(1 << kb[d]));
+    g.nc[d] = (int)ncd[d];
     if (g.bound[d] >= 2) {
-      const double L = g.xmax[d] - g.xmin[d];
-      if (!(L >= rmax)) sc->err = NDSP_ERR_CELLS;
-      ncd = floor(L / cs);
-      if (!(ncd >= 1.0)) ncd = 1.0;
-      if (ncd > (double)(1 << mbmax)) { ncd = (double)(1 << mbmax); sc->grid_clamped = 1; }
-      g.nc[d] = (int)ncd;
       g.lo[d] = g.xmin[d];
-      g.w[d] = L / g.nc[d];
+      g.w[d] = (g.xmax[d] - g.xmin[d]) / g.nc[d];
     } else {
       const double ext = sc->bbox_hi[d] - sc->bbox_lo[d];
-      ncd = floor(ext / cs) + 1.0;
-      double w = cs;
-      if (!(ncd <= (double)(1 << mbmax))) {
-        ncd = (double)(1 << mbmax);
-        w = fmax(cs, ext / (ncd - 1.0) * (1.0 + 1.0e-12));
-        sc->grid_clamped = 1;
-      }
-      g.nc[d] = (int)ncd;
       g.lo[d] = sc->bbox_lo[d];
-      g.w[d] = w;
+      g.w[d] = !clamped[d] ? cs
+               : g.nc[d] > 1 ? fmax(cs, ext / (g.nc[d] - 1.0) * (1.0 + 1.0e-12))
+                             : fmax(cs, ext * (1.0 + 1.0e-12));
     }
     g.inv_w[d] = 1.0 / g.w[d];
   }
-  int mx = max(g.nc[0], max(g.nc[1], g.nc[2]));
-  int mb = 0;
-  while ((1 << mb) < mx) mb++;
-  g.mbits = mb;
-  g.ncell = 1LL << (p.ndim * mb);
+  g.ncell = 1LL << tot;
+  {
+    // key layout: the bits every dimension has, then the next ones of the
+    // two longest (u < v), then the rest of the longest
+    int m1 = kb[0];
+    for (int d = 1; d < p.ndim; d++) m1 = min(m1, kb[d]);
+    int u = -1, v = -1;
+    for (int d = 0; d < p.ndim; d++)
+      if (kb[d] > m1) {
+        if (u < 0) u = d;
+        else v = d;
+      }
+    if (u < 0) u = 0;
+    if (v < 0) v = u;
+    g.km[0] = m1;
+    g.km[1] = v != u ? min(kb[u], kb[v]) - m1 : 0;
+    g.km[2] = u;
+    g.km[3] = v;
+  }
   int on = p.use_list;
   for (int d = 0; d < p.ndim; d++)
     if (g.bound[d] == 3 && !(rmax * (1.0 + p.skin) * 1.001 < g.xmax[d] - g.xmin[d])) on = 0;
@@ -366,13 +398,49 @@ __global__ void k_scatter(KArgs a) {
   b_scatter(a, VL{(int)blockIdx.x, (int)gridDim.x});
 }
 
+// ascending order of N values held in registers (bitonic network; every
+// index is fixed at compile time, so v stays in registers)
+template <int N>
+__device__ __forceinline__ void sort_regs(int (&v)[N]) {
+#pragma unroll
+  for (int k = 2; k <= N; k <<= 1)
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+      for (int i = 0; i < N; i++) {
+        const int l = i ^ j;
+        if (l > i) {
+          const int x = v[i], y = v[l];
+          const bool up = (i & k) == 0;
+          v[i] = up ? min(x, y) : max(x, y);
+          v[l] = up ? max(x, y) : min(x, y);
+        }
+      }
+}
+// cell [b, b+m) of sorted positions into ascending slot order by one lane,
+// m <= N: every entry is read before any is written back
+template <int N>
+__device__ __forceinline__ void sort_cell_lane(int* __restrict__ sidx, int b, int m) {
+  int v[N];
+#pragma unroll
+  for (int q = 0; q < N; q++) v[q] = q < m ? sidx[b + q] : INT_MAX;
+  sort_regs<N>(v);
+#pragma unroll
+  for (int q = 0; q < N; q++)
+    if (q < m) sidx[b + q] = v[q];
+}
+
 // make the order inside each cell ascending in slot index (deterministic,
-// and a stable sort by cell key overall), the whole wave sorting each cell
-// of its 64 cells in turn: a cell of at most 256 particles is ranked in
-// registers (each slot index counts the smaller ones of its cell), a larger
-// one is insertion-sorted by the first lane.  Cells of at most CS_LANE
-// particles (most of them) are ranked first, each by its own lane, all at once.
+// and a stable sort by cell key overall).  A lane sorts its own cell in
+// registers when the cell holds at most CS_LANE particles, or at most
+// CS_LANE2 while at least CS_LANE2_MIN cells of its wave are of that size
+// (the lanes then work side by side); the other cells are taken by the whole
+// wave in turn: a cell of at most 256 particles is ranked in registers (each
+// slot index counts the smaller ones of its cell), a larger one is
+// insertion-sorted by the first lane.
 #define CS_LANE 16
+#define CS_LANE2 64
+#define CS_LANE2_MIN 4
 __device__ __forceinline__ void b_cellsort_wave(KArgs a, const Grid* __restrict__ gp, VL vl) {
   const long long c = (long long)vl.b * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
@@ -381,20 +449,11 @@ __device__ __forceinline__ void b_cellsort_wave(KArgs a, const Grid* __restrict_
     b = a.cstart[c];
     m = a.cstart[c + 1] - b;
   }
-  if (m > 1 && m <= CS_LANE) {
-    // every entry is read before any is written back
-    int v[CS_LANE];
-#pragma unroll
-    for (int q = 0; q < CS_LANE; q++) v[q] = q < m ? a.sidx[b + q] : INT_MAX;
-#pragma unroll
-    for (int q = 0; q < CS_LANE; q++) {
-      int r = 0;
-#pragma unroll
-      for (int t = 0; t < CS_LANE; t++) r += v[t] < v[q] ? 1 : 0;
-      if (q < m) a.sidx[b + r] = v[q];
-    }
-  }
-  unsigned long long todo = __ballot(m > CS_LANE);
+  if (m > 1 && m <= CS_LANE) sort_cell_lane<CS_LANE>(a.sidx, b, m);
+  const bool mid = m > CS_LANE && m <= CS_LANE2;
+  const bool lanes_mid = __popcll(__ballot(mid)) >= CS_LANE2_MIN;
+  if (lanes_mid && mid) sort_cell_lane<CS_LANE2>(a.sidx, b, m);
+  unsigned long long todo = __ballot(m > CS_LANE && !(lanes_mid && mid));
   while (todo) {
     const int src = __ffsll((long long)todo) - 1;
     todo &= todo - 1ull;
@@ -671,8 +730,9 @@ __device__ __forceinline__ double image_r2(const Grid& g, const double* xa, cons
     double dx = xa[d] - xb[d];
     const unsigned kd = (code >> (2 * d)) & 3u;
     // kind 1: the image left of xmin (at x_j - L); kind 2: right of xmax
-    if (kd == 1u) dx = dx + (g.xmax[d] - g.xmin[d]);
-    else if (kd == 2u) dx = dx - (g.xmax[d] - g.xmin[d]);
+    // (selected without a branch; adding 0 leaves dx as it is)
+    const double L = g.xmax[d] - g.xmin[d];
+    dx = dx + (kd == 1u ? L : kd == 2u ? -L : 0.0);
     r2 = r2 + dx * dx;
   }
   return r2;
@@ -913,13 +973,15 @@ __device__ __forceinline__ void b_nblist(KArgs a, Phys p, const Grid* __restrict
       if (act && r2 >= 0.0) cell_range(g, 0, xi[0], NDIM >= 2 ? sqrt(r2) : rad, lx, hx);
       lx = wave_min_i(lx);
       hx = wave_max_i(hx);
+      // the row's key bits (y and z parts of cell_key)
+      const unsigned kyz = NDIM == 1 ? 0u : key_part(g, 1, sy) | (NDIM == 3 ? key_part(g, 2, sz) : 0u);
       for (int cx0 = lx; cx0 <= hx; cx0 += 64) {
         const int nadd = min(64, hx - cx0 + 1);
         if (ncl + nadd > NBL_CELLS) process_batch();
         if (lane < nadd) {
           int kx = 0;
           const int sx = cell_image(g, 0, cx0 + lane, kx);
-          ckey[ncl + lane] = sx >= 0 ? cell_key(g, sx, sy, sz) : -1;
+          ckey[ncl + lane] = sx >= 0 ? (int)(key_part(g, 0, sx) | kyz) : -1;
           ccode[ncl + lane] = (int)list_code(kx, ky, kz);
         }
         ncl += nadd;
@@ -2842,21 +2904,22 @@ static int check_err(ndsp_ctx* c, const DScal& h) {
 }
 
 // set_linklist: boundary, statistics, grid, counting sort, density view
-// Morton key-space capacity (bits) for a grid of the current h: 2^ndim times
-// the key space of the present grid, at most 2^28 keys
+// Morton key-space capacity (bits) for a grid of the current h: the key
+// bits of every dimension (its cell count rounded up to a power of two) plus
+// extra_bits each, at most 2^28 keys
 static int size_key_space(ndsp_ctx* c, const DScal& hs, int extra_bits) {
   const double rmax = c->p.radkern * hs.hmax_bin;
   if (!(rmax > 0.0) || !std::isfinite(rmax)) return set_err(NDSP_ERR_CELLS, "link: max h <= 0");
   const double cs = rmax / c->p.cell_sub;
-  int mx = 1;
+  int bits = 0;
   for (int d = 0; d < c->p.ndim; d++) {
     const double ext = c->ibound[d] >= 2 ? c->xmax[d] - c->xmin[d] : hs.bbox_hi[d] - hs.bbox_lo[d];
-    const double ncd = std::floor(ext / cs) + 1.0;
-    mx = std::max(mx, (int)std::min(ncd, 1.0e8));
+    const double ncd = std::min(std::floor(ext / cs) + 1.0, 1.0e8);
+    int kb = 0;
+    while ((double)(1 << kb) < ncd) kb++;
+    bits += kb + extra_bits;
   }
-  int mb = 0;
-  while ((1 << mb) < mx) mb++;
-  const int bits = std::min(28, c->p.ndim * (mb + extra_bits));
+  bits = std::min(28, bits);
   if (bits == c->cap_bits) return 0;
   const long long cap = (1LL << bits) + 1;
   if (c->a.cstart) hipFree(c->a.cstart);

@@ -13,7 +13,11 @@
 struct Grid {
   int ndim;
   int nc[3];
-  int mbits;         // bits per dimension of the Morton cell key
+  int kb[3];         // key bits per dimension (2^kb[d] >= nc[d])
+  // layout of the key (cell_key): km[0] bits interleaved over every
+  // dimension, then km[1] over dimensions km[2] < km[3], then the rest of
+  // the longest
+  int km[4];
   int bound[3];      // 0 open, 2 reflecting, 3 periodic
   double lo[3];      // grid origin
   double w[3];       // cell width
@@ -24,7 +28,12 @@ struct Grid {
 
 // Morton (bit-interleaved) key of cell (cx, cy, cz): consecutive keys are
 // mostly adjacent cells, so 64 consecutive particles of the sorted order form
-// a compact group (x bit lowest).
+// a compact group (x bit lowest).  Dimensions may hold different numbers of
+// key bits kb[d] (an elongated box): the bits all dimensions have are
+// interleaved first, then those the two longer ones have, then the rest of
+// the longest (no branch: a dimension without bits at a level adds zeros),
+// so the key space is 2^(kb[0]+kb[1]+kb[2]) and equal kb give the plain
+// interleave.
 __host__ __device__ __forceinline__ unsigned spread_bits3(unsigned x) {
   x &= 0x3ffu;
   x = (x | (x << 16)) & 0x030000FFu;
@@ -41,10 +50,30 @@ __host__ __device__ __forceinline__ unsigned spread_bits2(unsigned x) {
   x = (x | (x << 1)) & 0x55555555u;
   return x;
 }
+// the key bits cell coordinate c of dimension d contributes (a key is the OR
+// of its dimensions' parts)
+__host__ __device__ __forceinline__ unsigned key_part(const Grid& g, int d, int c) {
+  const unsigned x = (unsigned)c;
+  if (g.ndim == 1) return x;
+  const int m1 = g.km[0], m2 = g.km[1], nd = g.ndim;
+  const unsigned M1 = (1u << m1) - 1u;
+  unsigned k = (nd == 3 ? spread_bits3(x & M1) : spread_bits2(x & M1)) << d;
+  // the two dimensions with the most bits (u = km[2] < v = km[3], or the
+  // same one twice): their next m2 bits interleaved, then the bits only the
+  // longest has (a dimension without bits at a level adds zeros)
+  const int u = g.km[2], v = g.km[3];
+  if (d == u || d == v) {
+    const unsigned M2 = (1u << m2) - 1u;
+    if (u != v) k |= spread_bits2((x >> m1) & M2) << (nd * m1 + (d == v ? 1 : 0));
+    k |= (x >> (m1 + m2)) << (nd * m1 + 2 * m2);
+  }
+  return k;
+}
 __host__ __device__ __forceinline__ int cell_key(const Grid& g, int cx, int cy, int cz) {
-  if (g.ndim == 3) return (int)(spread_bits3(cx) | (spread_bits3(cy) << 1) | (spread_bits3(cz) << 2));
-  if (g.ndim == 2) return (int)(spread_bits2(cx) | (spread_bits2(cy) << 1));
-  return cx;
+  if (g.ndim == 1) return cx;
+  unsigned k = key_part(g, 0, cx) | key_part(g, 1, cy);
+  if (g.ndim == 3) k |= key_part(g, 2, cz);
+  return (int)k;
 }
 
 // Image of a cell at unwrapped coordinate c along one dimension.
