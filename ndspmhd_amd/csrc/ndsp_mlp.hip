@@ -192,12 +192,12 @@ __global__ void k_mlp_to_bf16(const float* __restrict__ w, uint16_t* __restrict_
 typedef short mlp_bf16x4 __attribute__((ext_vector_type(4)));
 typedef float mlp_f32x16 __attribute__((ext_vector_type(16)));
 
-// tanh-form GELU with the hardware exponential and reciprocal
+// tanh-form GELU with the hardware exponential and reciprocal:
+// 0.5 x (1 + tanh u) = x / (1 + exp(-2u)); for very negative u the
+// exponential overflows to inf and the quotient takes its limit 0
 __device__ __forceinline__ float gelu_fast(float x) {
   const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
-  const float e = __expf(-2.0f * fabsf(u));
-  const float t = (1.0f - e) * __builtin_amdgcn_rcpf(1.0f + e);
-  return 0.5f * x * (1.0f + copysignf(t, u));
+  return x * __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
 }
 
 // per-lane weights of the transposed layers (see above)
@@ -251,18 +251,21 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const uint16_t* __restrict__ 
   MlpLaneW w;
   mlp_lane_weights(wbf, w);
   const int lane = threadIdx.x & 63, i = lane & 31, hb = lane >> 5;
-  const long long base = ((long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63)) ;
-  for (int r = 0; r < 2; r++) {
-    const long long p = base + 32 * r + i;
-    float x[4] = {0.f, 0.f, 0.f, 0.f};
-    if (p < B)
-      for (int j = 0; j < 4; j++) {
-        const int k = 4 * hb + j;
-        if (k < NIN) x[j] = X[p * ldx + k];
-      }
-    const float out = mlp_mfma_block(w, x);
-    if (hb == 0 && p < B) y[p] = out;
-  }
+  // each wave takes 64 samples at a time, strided over the grid (the loop
+  // bound is uniform in the wave, so every lane reaches every MFMA)
+  const long long step = (long long)gridDim.x * blockDim.x;
+  for (long long base = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < B; base += step)
+    for (int r = 0; r < 2; r++) {
+      const long long p = base + 32 * r + i;
+      float x[4] = {0.f, 0.f, 0.f, 0.f};
+      if (p < B)
+        for (int j = 0; j < 4; j++) {
+          const int k = 4 * hb + j;
+          if (k < NIN) x[j] = X[p * ldx + k];
+        }
+      const float out = mlp_mfma_block(w, x);
+      if (hb == 0 && p < B) y[p] = out;
+    }
 }
 
 // the seed of the h-rho iteration after the predictor: features formed from
@@ -273,50 +276,53 @@ __global__ __launch_bounds__(256) void k_mlp_seed_mfma(const uint16_t* __restric
   MlpLaneW w;
   mlp_lane_weights(wbf, w);
   const int lane = threadIdx.x & 63, i = lane & 31, hb = lane >> 5;
-  const long long base = ((long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63));
   const float dtf = (float)f.dt, nexpf = (float)f.nexp, inv_nexp = (float)(1.0 / f.nexp);
   const float dndimf = (float)f.dndim, inv_hfact = (float)(1.0 / f.hfact);
-  for (int r = 0; r < 2; r++) {
-    const long long p = base + 32 * r + i;
-    const bool ok = p < f.n;
-    float x[4] = {0.f, 0.f, 0.f, 0.f};
-    double hpred = 0.0;
-    // features in FP32 (network inputs; the same arithmetic forms the
-    // training rows), the half-wave taking features 0-3 or 4-5
-    if (ok) {
-      hpred = f.h[p];
-      const float hin = (float)f.hin[p], rho = (float)f.rho[p];
-      if (hb == 0) {
-        x[0] = 100.0f * dtf * (float)f.dhdtin[p] * __builtin_amdgcn_rcpf(hin);
-        x[1] = rho > 0.0f ? 100.0f * dtf * (float)f.drhodt[p] * __builtin_amdgcn_rcpf(rho) : 0.0f;
-        x[2] = (float)(1000.0 * f.corr[p]);
-        x[3] = ((float)f.numneigh[p] - nexpf) * inv_nexp;
-      } else {
-        x[0] = (float)f.nits[p] - 1.5f;
-        x[1] = rho > 0.0f
-                   ? 1000.0f * (hin * exp2f(dndimf * __log2f(rho * __builtin_amdgcn_rcpf((float)f.m[p]))) * inv_hfact -
-                                1.0f)
-                   : 0.0f;
+  // 64 particles a wave at a time, strided over the grid, so that the lane
+  // weights are loaded once per wave (the loop bound is uniform in the wave)
+  const long long step = (long long)gridDim.x * blockDim.x;
+  for (long long base = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < f.n; base += step)
+    for (int r = 0; r < 2; r++) {
+      const long long p = base + 32 * r + i;
+      const bool ok = p < f.n;
+      float x[4] = {0.f, 0.f, 0.f, 0.f};
+      double hpred = 0.0;
+      // features in FP32 (network inputs; the same arithmetic forms the
+      // training rows), the half-wave taking features 0-3 or 4-5
+      if (ok) {
+        hpred = f.h[p];
+        const float hin = (float)f.hin[p], rho = (float)f.rho[p];
+        if (hb == 0) {
+          x[0] = 100.0f * dtf * (float)f.dhdtin[p] * __builtin_amdgcn_rcpf(hin);
+          x[1] = rho > 0.0f ? 100.0f * dtf * (float)f.drhodt[p] * __builtin_amdgcn_rcpf(rho) : 0.0f;
+          x[2] = (float)(1000.0 * f.corr[p]);
+          x[3] = ((float)f.numneigh[p] - nexpf) * inv_nexp;
+        } else {
+          x[0] = (float)f.nits[p] - 1.5f;
+          x[1] = rho > 0.0f
+                     ? 1000.0f * (hin * exp2f(dndimf * __log2f(rho * __builtin_amdgcn_rcpf((float)f.m[p]))) * inv_hfact -
+                                  1.0f)
+                     : 0.0f;
+        }
+      }
+      const float out = mlp_mfma_block(w, x);
+      if (!ok) continue;
+      if (f.X) {
+        float* row = f.X + p * MLP_LDX;
+        if (hb == 0) {
+          for (int j = 0; j < 4; j++) row[j] = x[j];
+        } else {
+          row[4] = x[0];
+          row[5] = x[1];
+          row[MLP_COL_HPRED] = (float)hpred;
+        }
+      }
+      if (f.apply && hb == 0 && !(f.itype && f.itype[p] == 1)) {
+        float yy = out / MLP_TSCALE;
+        yy = fminf(fmaxf(yy, -0.1f), 0.1f);
+        if (isfinite(yy) && yy != 0.0f && hpred > 0.0) f.h[p] = hpred * (double)__expf(yy);
       }
     }
-    const float out = mlp_mfma_block(w, x);
-    if (!ok) continue;
-    if (f.X) {
-      float* row = f.X + p * MLP_LDX;
-      if (hb == 0) {
-        for (int j = 0; j < 4; j++) row[j] = x[j];
-      } else {
-        row[4] = x[0];
-        row[5] = x[1];
-        row[MLP_COL_HPRED] = (float)hpred;
-      }
-    }
-    if (f.apply && hb == 0 && !(f.itype && f.itype[p] == 1)) {
-      float yy = out / MLP_TSCALE;
-      yy = fminf(fmaxf(yy, -0.1f), 0.1f);
-      if (isfinite(yy) && yy != 0.0f && hpred > 0.0) f.h[p] = hpred * (double)__expf(yy);
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -324,6 +330,8 @@ __global__ __launch_bounds__(256) void k_mlp_seed_mfma(const uint16_t* __restric
 // ---------------------------------------------------------------------------
 
 static inline unsigned nb(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+// workgroups of the matrix-core inference: four waves each, a few per CU
+static constexpr unsigned MLP_MFMA_BLOCKS = 1024;
 
 hipError_t mlp_alloc(MlpState& s, int n) {
   hipError_t e;
@@ -364,11 +372,13 @@ hipError_t mlp_set_params(MlpState& s, const float* host, hipStream_t st) {
 
 
 void mlp_launch_seed_mfma(MlpState& s, const MlpSeedFields& f, hipStream_t st) {
-  hipLaunchKernelGGL(k_mlp_seed_mfma, dim3(nb(f.n, 256)), dim3(256), 0, st, (const uint16_t*)s.wbf, f);
+  hipLaunchKernelGGL(k_mlp_seed_mfma, dim3(std::min(nb(f.n, 256), MLP_MFMA_BLOCKS)), dim3(256), 0, st,
+                     (const uint16_t*)s.wbf, f);
 }
 
 void mlp_launch_infer(MlpState& s, const float* X, int ldx, int B, float* y, hipStream_t st) {
-  hipLaunchKernelGGL(k_mlp_infer, dim3(nb(B, 256)), dim3(256), 0, st, (const uint16_t*)s.wbf, X, ldx, B, y);
+  hipLaunchKernelGGL(k_mlp_infer, dim3(std::min(nb(B, 256), MLP_MFMA_BLOCKS)), dim3(256), 0, st,
+                     (const uint16_t*)s.wbf, X, ldx, B, y);
 }
 
 void mlp_launch_forward(MlpState& s, const float* X, int ldx, int B, float* y, hipStream_t st) {
