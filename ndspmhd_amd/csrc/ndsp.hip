@@ -766,7 +766,7 @@ __device__ __forceinline__ int wave_max_i(int v) {
 // image test to those.  A pair k-j with j in G(k) and k not in G(j) is
 // appended to the extras of j, which fill j's row from its end
 // (k_nblist_sort moves them into place).
-#define NBL_CELLS 256
+#define NBL_CELLS 128
 template <int NDIM>
 __device__ __forceinline__ void b_nblist(KArgs a, Phys p, const Grid* __restrict__ gp, VL vl) {
   __shared__ double4 sraw[128];
@@ -999,7 +999,11 @@ __device__ __forceinline__ void b_nblist(KArgs a, Phys p, const Grid* __restrict
   }
 }
 template <int NDIM>
-__global__ __launch_bounds__(64) void k_nblist(KArgs a, Phys p, const Grid* __restrict__ gp) {
+// four waves per SIMD (a few registers spill; the LDS of a wave fits 4.5):
+// the builder waits on its candidate fetches, and 3 -> 4 waves took a
+// 128^3 build from 2.31 to 2.04 ms
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_nblist(KArgs a, Phys p,
+                                                                                        const Grid* __restrict__ gp) {
   b_nblist<NDIM>(a, p, gp, VL{(int)blockIdx.x, (int)gridDim.x});
 }
 
@@ -2710,7 +2714,9 @@ NDSP_MK(km_pack_density) { NDSP_MEMBER b_pack_density<PT>(M.a, M.p, vl); }
 template <typename PT>
 NDSP_MK(km_pack_force) { NDSP_MEMBER b_pack_force<PT>(M.a, M.p, vl); }
 template <int NDIM>
-__global__ __launch_bounds__(64) void km_nblist(const MArgs* __restrict__ ma, const int* __restrict__ off, int nm) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void km_nblist(const MArgs* __restrict__ ma,
+                                                                                         const int* __restrict__ off,
+                                                                                         int nm) {
   NDSP_MEMBER b_nblist<NDIM>(M.a, M.p, M.gdev, vl);
 }
 template <int NDIM, typename PT, bool DUST, int TB>
