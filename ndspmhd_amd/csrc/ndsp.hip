@@ -633,10 +633,11 @@ __device__ __forceinline__ unsigned inverse_code(unsigned code) {
 // calling body(kk, r, xj_img, flip, code) with r = load(kk) (a struct holding
 // the FP64 position and 1/h in r.p and whatever the pair sum reads) and the
 // image coordinates of the ghost arithmetic (so the exact FP64 tests see the
-// values of the cell scan).  Entries come four per load; the records of the
-// next PF entries (1 or 2) are fetched while the current one is processed, so
-// a wavefront's gathers overlap its arithmetic (two ahead where the pair
-// arithmetic is light and the records small).
+// values of the cell scan).  Entries come four per load; the record of entry
+// m + PF (PF 1 or 2) is fetched while entry m is processed, so a wavefront's
+// gathers overlap its arithmetic (two ahead where the pair arithmetic is
+// light and the records small).  The loop is unrolled over the PF + 1 record
+// buffers, which take turns as the current one, so no record is copied.
 template <int NDIM, int PF = 1, typename LOAD, typename F>
 __device__ __forceinline__ void for_each_listed_pf(const KArgs& a, const Grid& g, int k, int cnt, LOAD&& load,
                                                    F&& body) {
@@ -647,20 +648,19 @@ __device__ __forceinline__ void for_each_listed_pf(const KArgs& a, const Grid& g
   // entries are read four at a time into an eight-entry window (c0..c3 the
   // current ones, n0..n3 the next group) that shifts by one per entry, so
   // that no entry is picked by a run-time index (an indexed uint4 would live
-  // in scratch memory); the next record is fetched one entry ahead
+  // in scratch memory)
   const uint4 g0 = L4[0];
   const uint4 g1 = ng > 1 ? L4[1] : make_uint4(0u, 0u, 0u, 0u);
   unsigned c0 = g0.x, c1 = g0.y, c2 = g0.z, c3 = g0.w, n0 = g1.x, n1 = g1.y, n2 = g1.z, n3 = g1.w;
-  auto r = load((int)(c0 >> 7));
-  auto r1 = r;
-  if (PF == 2) r1 = load((int)((cnt > 1 ? c1 : c0) >> 7));
-  for (int m = 0; m < cnt; m++) {
+  int m = 0;
+  // entry m, its record in cur; the record of entry m + PF goes into nxt
+  auto visit = [&](const auto& cur, auto& nxt) {
     const unsigned e = c0;
     const unsigned en = m + PF < cnt ? (PF == 2 ? c2 : c1) : e;
-    const auto rn = load((int)(en >> 7));
+    nxt = load((int)(en >> 7));
     const int kk = (int)(e >> 7);
     const int code = (int)(e & 63u);
-    double xj[3] = {r.p.x, r.p.y, r.p.z};
+    double xj[3] = {cur.p.x, cur.p.y, cur.p.z};
     const int flip[3] = {0, 0, 0};
     bool ok = true;
     if (code) {
@@ -678,18 +678,35 @@ __device__ __forceinline__ void for_each_listed_pf(const KArgs& a, const Grid& g
         }
       }
     }
-    if (ok) body(kk, r, xj, flip, code);
-    if (PF == 2) {
-      r = r1;
-      r1 = rn;
-    } else {
-      r = rn;
-    }
+    if (ok) body(kk, cur, xj, flip, code);
     c0 = c1; c1 = c2; c2 = c3; c3 = n0; n0 = n1; n1 = n2; n2 = n3;
     if ((m & 3) == 3) {
       const int gi2 = (m >> 2) + 2;
       const uint4 t = gi2 < ng ? L4[gi2] : make_uint4(0u, 0u, 0u, 0u);
       n0 = t.x; n1 = t.y; n2 = t.z; n3 = t.w;
+    }
+    m++;
+  };
+  using R = decltype(load(0));
+  R rA = load((int)(c0 >> 7));
+  if constexpr (PF == 1) {
+    R rB;
+    for (;;) {
+      if (m >= cnt) break;
+      visit(rA, rB);
+      if (m >= cnt) break;
+      visit(rB, rA);
+    }
+  } else {
+    R rB = load((int)((cnt > 1 ? c1 : c0) >> 7));
+    R rC;
+    for (;;) {
+      if (m >= cnt) break;
+      visit(rA, rC);
+      if (m >= cnt) break;
+      visit(rB, rA);
+      if (m >= cnt) break;
+      visit(rC, rB);
     }
   }
 }
