@@ -3668,7 +3668,10 @@ static int step_part_a(ndsp_ctx* c) {
 // wrap, next dt; online training of the h predictor when enabled.
 static int step_part_b(ndsp_ctx* c) {
   MlpState& ml = c->mlp;
-  if (ml.enabled) hipLaunchKernelGGL(k_mlp_target, dim3(nblk(c->n)), dim3(BS), 0, c->st, c->a, ml.X);
+  if (ml.enabled) {
+    TScope t(c, "mlp_target");
+    hipLaunchKernelGGL(k_mlp_target, dim3(nblk(c->n)), dim3(BS), 0, c->st, c->a, ml.X);
+  }
   if (int e = do_rates(c)) return e;
   {
     TScope t(c, "correct");
