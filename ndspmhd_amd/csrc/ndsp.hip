@@ -759,6 +759,14 @@ __device__ __forceinline__ double image_r2(const Grid& g, const double* xa, cons
 // that they do not evict the neighbour records the gathers reuse from L2
 __device__ __forceinline__ void st_stream(double* p, double v) { __builtin_nontemporal_store(v, p); }
 
+// a value every lane of the wave holds, moved to scalar registers (frees
+// vector registers in the register-bound list builder)
+__device__ __forceinline__ double wave_uniform(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(unsigned)(b & 0xffffffffll));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
 __device__ __forceinline__ int wave_min_i(int v) {
   for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
   return v;
@@ -777,10 +785,10 @@ __device__ __forceinline__ int wave_max_i(int v) {
 // and then streams the batch's candidates 64 at a time through LDS (the next
 // 64 are fetched while the current ones are tested): FP64 raw positions for
 // the exact test and FP32 image positions relative to the wave's first
-// particle for a prefilter (relative margin 1e-4 on r^2, far above the FP32
-// rounding of the separations within a compact group).  Each lane marks the
-// candidates that pass the prefilter, then applies the exact FP64
-// image test to those.  A pair k-j with j in G(k) and k not in G(j) is
+// particle for a screening test.  A decision of the FP32 test taken outside
+// a narrow band around the radius (its width bounded by the FP32 rounding,
+// see below) is the FP64 test's decision; a candidate inside the band takes
+// the exact FP64 image test.  A pair k-j with j in G(k) and k not in G(j) is
 // appended to the extras of j, which fill j's row from its end
 // (k_nblist_sort moves them into place).
 #define NBL_CELLS 128
@@ -803,14 +811,13 @@ __device__ __forceinline__ void b_nblist(KArgs a, Phys p, const Grid* __restrict
   const Grid g = *gp;
   const double s1 = 1.0 + p.skin;
   const double lim2 = p.radkern2 * s1 * s1 * (1.0 + 1.0e-12);
-  const float lim2f = (float)(p.radkern2 * s1 * s1 * (1.0 + 1.0e-4));
   const double4 pk = a.P4[act ? k : 0];
   const double xi[3] = {pk.x, pk.y, pk.z};
   const double hi21 = pk.w * pk.w;
-  const float hi2f = (float)hi21;
   const double rad = act ? p.radkern * s1 * (1.0 + 1.0e-9) / pk.w : 0.0;
   const double rad2 = rad * rad * (1.0 + 1.0e-12);
-  const double org[3] = {__shfl(xi[0], 0), __shfl(xi[1], 0), __shfl(xi[2], 0)};
+  const double org[3] = {wave_uniform(__shfl(xi[0], 0)), wave_uniform(__shfl(xi[1], 0)),
+                          wave_uniform(__shfl(xi[2], 0))};
   const float xr0 = (float)(xi[0] - org[0]);
   const float xr1 = NDIM >= 2 ? (float)(xi[1] - org[1]) : 0.f;
   const float xr2 = NDIM >= 3 ? (float)(xi[2] - org[2]) : 0.f;
@@ -825,56 +832,97 @@ __device__ __forceinline__ void b_nblist(KArgs a, Phys p, const Grid* __restrict
     double rw = rad;
     for (int o = 32; o > 0; o >>= 1) rw = fmax(rw, __shfl_xor(rw, o));
     rw *= 1.0 + 1.0e-9;
-    rw2 = rw * rw;
+    rw2 = wave_uniform(rw * rw);
     for (int d = 0; d < 3; d++) {
       double lo_ = act ? xi[d] : DBL_MAX, hi_ = act ? xi[d] : -DBL_MAX;
       for (int o = 32; o > 0; o >>= 1) {
         lo_ = fmin(lo_, __shfl_xor(lo_, o));
         hi_ = fmax(hi_, __shfl_xor(hi_, o));
       }
-      gbox_lo[d] = lo_;
-      gbox_hi[d] = hi_;
+      gbox_lo[d] = wave_uniform(lo_);
+      gbox_hi[d] = wave_uniform(hi_);
     }
   }
   int fill = 0;  // staged candidates awaiting the test (wave-uniform)
 
-  // test the staged candidates [0, m) against this lane's particle
-  const float rl2f = lim2f / hi2f;  // squared list radius of this particle
-  auto test = [&](int m) {
-    if (!act) return;
-    unsigned long long mask = 0ull;
-#pragma unroll 1
-    for (int t0 = 0; t0 < m; t0 += 8) {
-      // 8 candidates with a constant bit each (entries past m are masked)
-      unsigned m8 = 0u;
-#pragma unroll
-      for (int u = 0; u < 8; u++) {
-        const float4 c = srel[t0 + u];
-        const float dx = xr0 - c.x, dy = xr1 - c.y, dz = xr2 - c.z;
-        const float r2f = dx * dx + dy * dy + dz * dz;
-        m8 |= r2f < rl2f ? (1u << u) : 0u;
-      }
-      mask |= (unsigned long long)m8 << t0;
+  // FP32 screening.  Positions are rounded to FP32 relative to the origin
+  // org; every staged candidate (and particle of the group) lies within Mg
+  // of it (the group box's farthest corner plus the largest radius).  With
+  // u = 2^-24, the FP32 squared separation misses r^2 by at most
+  // 5u r^2 + 7u Mg r, i.e. near a radius R by a relative 5u + 7u Mg/R; the
+  // band eps(R) = 4e-6 (4 + Mg/R) is more than twice that, thresholds'
+  // rounding included, so a decision outside the band is the FP64 one.
+  float Mg = 0.0f;
+  {
+    double m2 = 0.0;
+    for (int d = 0; d < NDIM; d++) {
+      const double md = fmax(org[d] - gbox_lo[d], gbox_hi[d] - org[d]);
+      m2 += md * md;
     }
-    if (m < 64) mask &= (1ull << m) - 1ull;
+    Mg = (float)((sqrt(m2) + sqrt(rw2)) * (1.0 + 1.0e-6));
+  }
+  const double rl2 = act ? lim2 / hi21 : 0.0;  // squared list radius of this particle
+  const float eps_k = act ? 4.0e-6f * (4.0f + Mg / (float)sqrt(rl2)) : 0.0f;
+  const float rl2_out = (float)rl2 * (1.0f + eps_k), rl2_in = (float)rl2 * (1.0f - eps_k);
+  const float lim2_f = (float)lim2;
+  // lim2 * eps(R) for a candidate's radius R = sqrt(lim2) / w: band0 + band1 w
+  const float band0 = lim2_f * 1.6e-5f, band1 = lim2_f * 4.0e-6f * Mg / (float)sqrt(lim2);
+  // test the staged candidates [0, m) against this lane's particle (every
+  // lane of the wave takes part; an idle lane has no candidates)
+  auto test = [&](int m) {
+    unsigned long long mask = 0ull;
+    if (act) {
+#pragma unroll 1
+      for (int t0 = 0; t0 < m; t0 += 8) {
+        // 8 candidates with a constant bit each (entries past m are masked)
+        unsigned m8 = 0u;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const float4 c = srel[t0 + u];
+          const float dx = xr0 - c.x, dy = xr1 - c.y, dz = xr2 - c.z;
+          const float r2f = dx * dx + dy * dy + dz * dz;
+          m8 |= r2f < rl2_out ? (1u << u) : 0u;
+        }
+        mask |= (unsigned long long)m8 << t0;
+      }
+      if (m < 64) mask &= (1ull << m) - 1ull;
+    }
     while (mask) {
       const int t = __builtin_ctzll(mask);
       mask &= mask - 1ull;
-      const double4 c = sraw[t];
-      const double xr[3] = {c.x, c.y, c.z};
       const unsigned e = se[t];
-      const double rr = image_r2<NDIM>(g, xi, xr, e & 63u);
-      if (!(rr * hi21 < lim2)) continue;
+      const float4 cr = srel[t];
+      const float dx = xr0 - cr.x, dy = xr1 - cr.y, dz = xr2 - cr.z;
+      const float r2f = dx * dx + dy * dy + dz * dz;
+      double rr = -1.0;
+      if (!(r2f < rl2_in)) {
+        const double4 c = sraw[t];
+        const double xr[3] = {c.x, c.y, c.z};
+        rr = image_r2<NDIM>(g, xi, xr, e & 63u);
+        if (!(rr * hi21 < lim2)) continue;
+      }
       if (cnt < a.maxnb) L[cnt] = e;
       else over = true;
       cnt++;
       const int kk = (int)(e >> 7);
-      // same operands as kk's own test of the opposite image of k (rr is
-      // symmetric); an image of k itself has its opposite in G(k) already
-      if (kk != k && !(rr * (c.w * c.w) < lim2)) {
-        const int slot = atomicAdd(&a.nbx[kk], 1);
-        if (slot < a.maxnb)
-          a.nbl[(size_t)kk * a.maxnb + (a.maxnb - 1 - slot)] = ((unsigned)k << 7) | inverse_code(e & 63u);
+      if (kk != k) {
+        // q - lim2 against the band lim2 * eps(R_kk), R_kk = rlim / w_kk
+        const float dq = r2f * (cr.w * cr.w) - lim2_f;
+        const float band = band0 + band1 * cr.w;
+        int kin = dq < -band ? 1 : dq > band ? 0 : -1;
+        if (kin < 0) {
+          const double4 c = sraw[t];
+          if (rr < 0.0) {
+            const double xr[3] = {c.x, c.y, c.z};
+            rr = image_r2<NDIM>(g, xi, xr, e & 63u);
+          }
+          kin = rr * (c.w * c.w) < lim2 ? 1 : 0;
+        }
+        if (!kin) {
+          const int slot = atomicAdd(&a.nbx[kk], 1);
+          if (slot < a.maxnb)
+            a.nbl[(size_t)kk * a.maxnb + (a.maxnb - 1 - slot)] = ((unsigned)k << 7) | inverse_code(e & 63u);
+        }
       }
     }
   };
@@ -940,7 +988,7 @@ __device__ __forceinline__ void b_nblist(KArgs a, Phys p, const Grid* __restrict
         const int pos = fill + __popcll(bal & ((1ull << lane) - 1ull));
         sraw[pos] = pj;
         srel[pos] = make_float4((float)(xj[0] - org[0]), NDIM >= 2 ? (float)(xj[1] - org[1]) : 0.f,
-                                NDIM >= 3 ? (float)(xj[2] - org[2]) : 0.f, 0.f);
+                                NDIM >= 3 ? (float)(xj[2] - org[2]) : 0.f, (float)pj.w);
         se[pos] = ent;
       }
       fill += __popcll(bal);

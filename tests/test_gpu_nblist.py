@@ -132,3 +132,46 @@ def test_lists_holding_both_images_of_a_neighbour(prec, tol):
     assert "nblist" in n0
     assert np.array_equal(c0[0], c1[0]) and np.array_equal(c0[1], c1[1])
     _close(s0, s1, tol)
+
+
+def test_list_lengths_at_the_fp32_screening_band():
+    """Each particle's h puts its list radius within 2e-6 (relative) of its
+    distance to one neighbour, inside the band where the builder's FP32
+    screening defers to the FP64 test: every list length (gather part plus
+    transposed extras) equals the count of the exact FP64 membership test
+    replayed on the host, image by image"""
+    import itertools
+    import ndspmhd_amd as nd
+    from ndspmhd_amd.setups import periodic_mhd_box
+    o, st = periodic_mhd_box(8, seed=21, vel_seed=22, vrms=0.3)
+    o = o.replace(maxdensits=0)
+    x = st["x"]
+    n = x.shape[0]
+    lim2 = 2.0 * 2.0 * 1.04 * 1.04 * (1.0 + 1.0e-12)  # radkern^2 (1+skin)^2 (1+1e-12)
+    shifts = np.array(list(itertools.product((-1.0, 0.0, 1.0), repeat=3)))  # box side 1
+    d = (x[:, None, None, :] - x[None, :, None, :]) + shifts[None, None, :, :]
+    r2 = d[..., 0] * d[..., 0]
+    r2 = r2 + d[..., 1] * d[..., 1]
+    r2 = r2 + d[..., 2] * d[..., 2]
+    rng = np.random.default_rng(23)
+    h0 = st["hh"][0]
+    h = np.empty(n)
+    for k in range(n):
+        # the neighbour whose distance is nearest the natural list radius
+        cand = r2[k][r2[k] > 0.0]
+        rr = cand[np.argmin(np.abs(cand - lim2 * h0 * h0))]
+        h[k] = np.sqrt(rr / lim2) * (1.0 + rng.uniform(-2.0e-6, 2.0e-6))
+    st["hh"][:] = h
+    hi = 1.0 / h
+    hi21 = hi * hi
+    ink = r2 * hi21[:, None, None] < lim2
+    inj = r2 * hi21[None, :, None] < lim2
+    expected = np.sum(ink | inj, axis=(1, 2))
+    near = np.sum(np.abs(r2 * hi21[:, None, None] / lim2 - 1.0) < 1.6e-5)
+    print("pairs within the screening band:", int(near), flush=True)
+    assert near >= n
+    sim = nd.Simulation(o, st, precision=1)
+    sim.derivs()
+    L = sim.get("nblist_len")
+    assert (L >= 0).all()
+    assert np.array_equal(L.astype(np.int64), expected), np.nonzero(L != expected)
