@@ -887,6 +887,11 @@ __device__ __forceinline__ void b_nblist(KArgs a, Phys p, const Grid* __restrict
       }
       if (m < 64) mask &= (1ull << m) - 1ull;
     }
+    // an extra's slot is reserved when it is found and written when the next
+    // one is found (or after the loop), so the atomic's latency overlaps the
+    // tests in between
+    int pend_row = -1, pend_slot = 0;
+    unsigned pend_ent = 0u;
     while (mask) {
       const int t = __builtin_ctzll(mask);
       mask &= mask - 1ull;
@@ -919,12 +924,16 @@ __device__ __forceinline__ void b_nblist(KArgs a, Phys p, const Grid* __restrict
           kin = rr * (c.w * c.w) < lim2 ? 1 : 0;
         }
         if (!kin) {
-          const int slot = atomicAdd(&a.nbx[kk], 1);
-          if (slot < a.maxnb)
-            a.nbl[(size_t)kk * a.maxnb + (a.maxnb - 1 - slot)] = ((unsigned)k << 7) | inverse_code(e & 63u);
+          if (pend_row >= 0 && pend_slot < a.maxnb)
+            a.nbl[(size_t)pend_row * a.maxnb + (a.maxnb - 1 - pend_slot)] = pend_ent;
+          pend_slot = atomicAdd(&a.nbx[kk], 1);
+          pend_row = kk;
+          pend_ent = ((unsigned)k << 7) | inverse_code(e & 63u);
         }
       }
     }
+    if (pend_row >= 0 && pend_slot < a.maxnb)
+      a.nbl[(size_t)pend_row * a.maxnb + (a.maxnb - 1 - pend_slot)] = pend_ent;
   };
 
   int ncl = 0;  // cells in the current batch (wave-uniform)
