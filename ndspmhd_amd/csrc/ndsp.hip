@@ -1712,6 +1712,9 @@ __device__ __forceinline__ void b_force(KArgs a, Phys p, const Grid* __restrict_
 
   PT vsigmax_l = PT(0);
   double dtc_l = 1.0e6;
+  // FP32 pair math: the largest vsig_dtc / min(h_i, h_j) = vsig_dtc max(1/h_i, 1/h_j)
+  // over the pairs, one division at the end instead of two reciprocals a pair
+  PT rdt_l = PT(0);
   unsigned long long npairs = 0;
   int err = 0;
   if (active) {
@@ -1811,7 +1814,6 @@ __device__ __forceinline__ void b_force(KArgs a, Phys p, const Grid* __restrict_
       const PT prj = fmax(j1.y, PT(0));
       const PT Prho2j = prj * rho21j;
       const PT csj = j1.z;
-      const PT hj = dust ? j4.x : p_rcp(hj1p);
       PT Bj[3] = {0, 0, 0}, psij = 0, dB[3] = {0, 0, 0};
       PT projBi = 0, projBj = 0, projdB = 0, projBrhoi = 0, projBrhoj = 0, Brho2j = 0, valfven2j = 0;
       PT Brhoj[3] = {0, 0, 0};
@@ -1825,7 +1827,8 @@ __device__ __forceinline__ void b_force(KArgs a, Phys p, const Grid* __restrict_
         projBrhoi = Brhoi[0] * dr[0] + Brhoi[1] * dr[1] + Brhoi[2] * dr[2];
         projBrhoj = Brhoj[0] * dr[0] + Brhoj[1] * dr[1] + Brhoj[2] * dr[2];
         const PT B2j = Bj[0] * Bj[0] + Bj[1] * Bj[1] + Bj[2] * Bj[2];
-        valfven2j = p_div(B2j, dust ? j4.y : rhoj);
+        // B^2 / rho_j: the reciprocal formed above serves in FP32
+        valfven2j = dust ? p_div(B2j, j4.y) : sizeof(PT) == 4 ? B2j * rho1j : p_div(B2j, rhoj);
         Brho2j = B2j * rho21j;
       }
       // signal velocities, src/ratesND_mhd.f90:1417-1482
@@ -1851,7 +1854,14 @@ __device__ __forceinline__ void b_force(KArgs a, Phys p, const Grid* __restrict_
       const PT vsigu = p_sqrt(fabs(pri - prj) * rhoav1);
       const PT vsigdtc = fmax(fmax(vsig, PT(0.5) * (vsigi + vsigj + beta * fabs(dvdotr))), vsigB);
       vsigmax_l = fmax(vsigmax_l, vsigdtc);
-      if (vsigdtc > PT(1.e-10)) dtc_l = fmin(dtc_l, (double)p_div((PT)fmin(hi, hj), vsigdtc));
+      if (vsigdtc > PT(1.e-10)) {
+        if constexpr (sizeof(PT) == 4) {
+          rdt_l = fmax(rdt_l, vsigdtc * fmax(hi1p, hj1p));
+        } else {
+          const PT hj = dust ? j4.x : p_rcp(hj1p);
+          dtc_l = fmin(dtc_l, (double)p_div((PT)fmin(hi, hj), vsigdtc));
+        }
+      }
       PT f[3] = {0, 0, 0};
       PT du = PT(0);
       // artificial dissipation
@@ -1984,6 +1994,7 @@ __device__ __forceinline__ void b_force(KArgs a, Phys p, const Grid* __restrict_
   }
   if (err) atomicCAS(&a.sc->err, 0, err);
   {
+    if (rdt_l > PT(0)) dtc_l = fmin(dtc_l, 1.0 / (double)rdt_l);
     double rv[3] = {(double)vsigmax_l, dtc_l, (double)npairs};
     const int rk[3] = {RED_MAX, RED_MIN, RED_SUM};
     block_partials<3>(rv, rk, a.red, a.rcap, vl);
