@@ -2385,19 +2385,50 @@ __global__ void k_conserved(KArgs a, Phys p, double* part) {
   if (t < NCONS) part[blockIdx.x * NCONS + t] = sm[t][0];
 }
 
-__global__ void k_conserved_final(const double* part, int nb, double* out, int n) {
+// folds the per-block partials part[nb][NQ] into v[NQ] (shared memory) in
+// an order fixed by nb alone, so that the result does not depend on timing:
+// thread t takes blocks t, t+256, ... in turn, then a butterfly over each
+// wave and the four waves in order.  One block of 256 threads.
+#define FOLD_T 256
+template <int NQ, class Comb, class Ident>
+__device__ __forceinline__ void fold_block_partials(const double* __restrict__ part, int nb, double* v, Comb comb,
+                                                    Ident ident) {
+  __shared__ double sw[FOLD_T / 64][NQ];
+  double acc[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) acc[q] = ident(q);
+  for (int b = threadIdx.x; b < nb; b += FOLD_T) {
+    const double* r = part + (size_t)b * NQ;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) acc[q] = comb(q, acc[q], r[q]);
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    double x = acc[q];
+    for (int o = 32; o > 0; o >>= 1) x = comb(q, x, __shfl_xor(x, o, 64));
+    acc[q] = x;
+  }
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; q++) sw[threadIdx.x >> 6][q] = acc[q];
+  }
+  __syncthreads();
+  if (threadIdx.x < NQ) {
+    const int q = threadIdx.x;
+    double x = sw[0][q];
+    for (int k = 1; k < FOLD_T / 64; k++) x = comb(q, x, sw[k][q]);
+    v[q] = x;
+  }
+}
+
+__global__ __launch_bounds__(FOLD_T) void k_conserved_final(const double* part, int nb, double* out, int n) {
+  __shared__ double v[NCONS];
+  fold_block_partials<NCONS>(
+      part, nb, v,
+      [](int q, double x, double y) { return q == 12 ? fmin(x, y) : q == 13 ? fmax(x, y) : x + y; },
+      [](int q) { return q == 12 ? DBL_MAX : q == 13 ? -DBL_MAX : 0.0; });
+  __syncthreads();
   if (threadIdx.x | blockIdx.x) return;
-  double v[NCONS];
-  for (int q = 0; q < NCONS; q++) v[q] = 0.0;
-  v[12] = DBL_MAX;
-  v[13] = -DBL_MAX;
-  for (int b = 0; b < nb; b++)
-    for (int q = 0; q < NCONS; q++) {
-      const double x = part[b * NCONS + q];
-      if (q == 12) v[q] = fmin(v[q], x);
-      else if (q == 13) v[q] = fmax(v[q], x);
-      else v[q] += x;
-    }
   v[3] = v[0] + v[2] + v[1];
   v[7] = sqrt(v[4] * v[4] + v[5] * v[5] + v[6] * v[6]);
   v[11] = sqrt(v[8] * v[8] + v[9] * v[9] + v[10] * v[10]);
@@ -2501,17 +2532,16 @@ __global__ __launch_bounds__(BS) void k_ev_sums(KArgs a, Phys p, double* __restr
   }
 }
 
-// folds the block partials (in block order) and forms the columns of the
-// .ev line in the order evwrite writes them (src/evwrite_mhd.f90:247-308)
-__global__ void k_ev_final(const double* __restrict__ part, int nb, int n, Phys p, double t, double dt,
-                           double* __restrict__ out) {
+// folds the block partials and forms the columns of the .ev line in the
+// order evwrite writes them (src/evwrite_mhd.f90:247-308)
+__global__ __launch_bounds__(FOLD_T) void k_ev_final(const double* __restrict__ part, int nb, int n, Phys p,
+                                                     double t, double dt, double* __restrict__ out) {
+  __shared__ double v[EV_NQ];
+  fold_block_partials<EV_NQ>(
+      part, nb, v, [](int q, double x, double y) { return ev_combine(q, x, y); },
+      [](int q) { return ev_identity(q); });
+  __syncthreads();
   if (threadIdx.x | blockIdx.x) return;
-  double v[EV_NQ];
-  for (int q = 0; q < EV_NQ; q++) {
-    double x = ev_identity(q);
-    for (int b = 0; b < nb; b++) x = ev_combine(q, x, part[(size_t)b * EV_NQ + q]);
-    v[q] = x;
-  }
   const double epot = 0.0;  // no gravity or external potentials on this path
   const double ekin = v[EV_EKIN], etherm = v[EV_ETHERM], emag = v[EV_EMAG];
   // iprterm 0: the thermal energy counts (src/evwrite_mhd.f90:247-248)
@@ -4519,7 +4549,7 @@ int ndsp_conserved(ndsp_ctx* c, double* out) {
   if (!c || !out) return set_err(NDSP_ERR_BAD_ARG, "null argument");
   const int nb = std::min(c->nb_part, (int)nblk(c->n));
   hipLaunchKernelGGL(k_conserved, dim3(nb), dim3(BS), 0, c->st, c->a, c->p, c->part);
-  hipLaunchKernelGGL(k_conserved_final, dim3(1), dim3(1), 0, c->st, (const double*)c->part, nb, c->cons_out, c->n);
+  hipLaunchKernelGGL(k_conserved_final, dim3(1), dim3(FOLD_T), 0, c->st, (const double*)c->part, nb, c->cons_out, c->n);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, c->cons_out, sizeof(double) * NCONS, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
@@ -4532,7 +4562,7 @@ int ndsp_ev_columns(ndsp_ctx* c, double t, double dt, double* cols, int32_t* nco
   const int nb = std::min(c->nb_part, (int)nblk(c->n));
   const int nc = c->p.imhd ? 25 : c->p.idust ? 14 : 13;
   hipLaunchKernelGGL(k_ev_sums, dim3(nb), dim3(BS), 0, c->st, c->a, c->p, c->part);
-  hipLaunchKernelGGL(k_ev_final, dim3(1), dim3(1), 0, c->st, (const double*)c->part, nb, c->n, c->p, t, dt,
+  hipLaunchKernelGGL(k_ev_final, dim3(1), dim3(FOLD_T), 0, c->st, (const double*)c->part, nb, c->n, c->p, t, dt,
                      c->cons_out);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(cols, c->cons_out, sizeof(double) * nc, hipMemcpyDeviceToHost, c->st));
