@@ -1864,6 +1864,9 @@ __device__ __forceinline__ void b_force(KArgs a, Phys p, const Grid* __restrict_
       }
       PT f[3] = {0, 0, 0};
       PT du = PT(0);
+      // dB/dt terms of the pair (dissipation and induction), summed in the
+      // pair precision before the accumulation
+      PT dbp[3] = {0, 0, 0};
       // artificial dissipation
       if (dust) {
         // src/ratesND_mhd.f90:2154-2206
@@ -1894,9 +1897,7 @@ __device__ __forceinline__ void b_force(KArgs a, Phys p, const Grid* __restrict_
         }
         if (mhd) {
           const PT fac = rhoi * mj * alphaB * termB * rhoav1;
-          dbx.add(fac * dB[0]);
-          dby.add(fac * dB[1]);
-          dbz.add(fac * dB[2]);
+          for (int c = 0; c < 3; c++) dbp[c] = fac * dB[c];
         }
         if (p.iener > 0) {
           PT vissv = PT(0);
@@ -1928,9 +1929,10 @@ __device__ __forceinline__ void b_force(KArgs a, Phys p, const Grid* __restrict_
         cby.add(mj * cb[1] * grkern);
         cbz.add(mj * cb[2] * grkern);
         const PT ind = mj * projBrhoi * grkerni;
-        dbx.add(-ind * dvel[0]);
-        dby.add(-ind * dvel[1]);
-        dbz.add(-ind * dvel[2]);
+        for (int c = 0; c < 3; c++) dbp[c] -= ind * dvel[c];
+        dbx.add(dbp[0]);
+        dby.add(dbp[1]);
+        dbz.add(dbp[2]);
         if (p.idivbzero >= 2) {
           const PT gpt = mj * (psii * rho21i * grkerni + psij * rho21j * grkernj);
           gpx.add(-gpt * dr[0]);
