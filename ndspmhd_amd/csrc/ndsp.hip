@@ -520,6 +520,11 @@ __global__ void k_pack_density(KArgs a, Phys p) {
   b_pack_density<PT>(a, p, VL{(int)blockIdx.x, (int)gridDim.x});
 }
 
+// quads of a force record: the physics variant of the force pass (MHD,
+// else dust, else hydro) fixes the layout
+__host__ __device__ __forceinline__ int fr_phys(const Phys& p) { return p.imhd ? 1 : p.idust ? 2 : 0; }
+__host__ __device__ constexpr int fr_slots(int phys) { return phys == 1 ? 4 : phys == 2 ? 5 : 3; }
+
 template <typename PT>
 __device__ __forceinline__ void b_pack_force(KArgs a, Phys p, VL vl) {
   int k = vl.b * blockDim.x + threadIdx.x;
@@ -530,17 +535,21 @@ __device__ __forceinline__ void b_pack_force(KArgs a, Phys p, VL vl) {
   // a fixed particle's h was replaced by its partner's after the iteration
   if (p.fixed_on && p.use_list && a.itype[i] == 1 && h > a.hcap[k]) atomicOr(&a.sc->list_invalid, 1);
   typedef typename V4<PT>::type T4;
-  ((T4*)a.G0)[k] = T4{(PT)a.f[F_VX][i], (PT)a.f[F_VY][i], (PT)a.f[F_VZ][i], (PT)a.f[F_M][i]};
-  ((T4*)a.G1)[k] = T4{(PT)a.f[F_RHO][i], (PT)a.f[F_PR][i], (PT)a.f[F_CS][i], (PT)a.f[F_GRADH][i]};
-  ((T4*)a.G2)[k] = T4{(PT)a.f[F_UU][i], (PT)a.f[F_A0][i], (PT)a.f[F_A1][i], (PT)a.f[F_A2][i]};
+  const int phys = fr_phys(p);
+  T4* R = (T4*)a.FR + (size_t)k * fr_slots(phys);
+  R[0] = T4{(PT)a.f[F_VX][i], (PT)a.f[F_VY][i], (PT)a.f[F_VZ][i], (PT)a.f[F_M][i]};
+  R[1] = T4{(PT)a.f[F_RHO][i], (PT)a.f[F_PR][i], (PT)a.f[F_CS][i], (PT)a.f[F_GRADH][i]};
+  R[2] = T4{(PT)a.f[F_UU][i], (PT)a.f[F_A0][i], (PT)a.f[F_A1][i], (PT)a.f[F_A2][i]};
+  if (phys == 1) R[3] = T4{(PT)a.f[F_BX][i], (PT)a.f[F_BY][i], (PT)a.f[F_BZ][i], (PT)a.f[F_PSI][i]};
   if (p.imhd) {
-    ((T4*)a.G3)[k] = T4{(PT)a.f[F_BX][i], (PT)a.f[F_BY][i], (PT)a.f[F_BZ][i], (PT)a.f[F_PSI][i]};
     // Bfield of conservative2primitive (src/conservative2primitive.f90:152),
     // kept for the outputs after the corrector has moved B on
     for (int d = 0; d < 3; d++) a.f[F_BFX + d][i] = a.f[F_BX + d][i];
   }
-  ((T4*)a.G4)[k] = T4{(PT)h, (PT)a.f[F_DENS][i], (PT)a.f[F_DUSTFRAC][i], (PT)a.f[F_RHOGAS][i]};
-  if (p.idust) ((PT*)a.G5)[k] = (PT)a.f[F_RHODUST][i];
+  if (phys == 2) {
+    R[3] = T4{(PT)h, (PT)a.f[F_DENS][i], (PT)a.f[F_DUSTFRAC][i], (PT)a.f[F_RHOGAS][i]};
+    R[4] = T4{(PT)a.f[F_RHODUST][i], PT(0), PT(0), PT(0)};
+  }
   if (p.fixed_on) {
     const bool fx = a.itype[i] == 1;
     ((T4*)a.GF)[k] = T4{(PT)a.f[fx ? F_MIMG : F_M][i], (PT)a.f[fx ? F_A0IMG : F_A0][i],
@@ -1697,15 +1706,11 @@ __device__ __forceinline__ void b_force(KArgs a, Phys p, const Grid* __restrict_
   const int k = xcd_tile(vl) * blockDim.x + threadIdx.x;
   const bool active = k < a.n;
   typedef typename V4<PT>::type T4;
-  const T4* G0 = (const T4*)a.G0;
-  const T4* G1 = (const T4*)a.G1;
-  const T4* G2 = (const T4*)a.G2;
-  const T4* G3 = (const T4*)a.G3;
-  const T4* G4 = (const T4*)a.G4;
-  const PT* G5 = (const PT*)a.G5;
-  const T4* GF = (const T4*)a.GF;
   constexpr bool mhd = PHYS == 1;
   constexpr bool dust = PHYS == 2;
+  constexpr int NS = fr_slots(PHYS);
+  const T4* FR = (const T4*)a.FR;
+  const T4* GF = (const T4*)a.GF;
   const PT ddq2 = (PT)p.ddq2table, dq2 = (PT)p.dq2table;
   const double R2 = p.radkern2;
   const PT beta = (PT)p.beta;
@@ -1723,7 +1728,9 @@ __device__ __forceinline__ void b_force(KArgs a, Phys p, const Grid* __restrict_
     const double xi[3] = {pk.x, pk.y, pk.z};
     const double hi1 = pk.w;
     const double hi21 = mul_exact(hi1, hi1);
-    const T4 g0 = G0[k], g1 = G1[k], g2 = G2[k], g4 = G4[k];
+    const T4* ri = FR + (size_t)k * NS;
+    const T4 g0 = ri[0], g1 = ri[1], g2 = ri[2];
+    const T4 g4 = dust ? ri[3] : T4{PT(0), PT(0), PT(0), PT(0)};
     const PT vi[3] = {g0.x, g0.y, g0.z};
     const PT mi = g0.w;
     const PT rhoi = g1.x, rho1i = p_rcp(rhoi), rho21i = rho1i * rho1i;
@@ -1731,18 +1738,19 @@ __device__ __forceinline__ void b_force(KArgs a, Phys p, const Grid* __restrict_
     const PT Prho2i = pri * rho21i;
     const PT csi = g1.z, gradhi = g1.w;
     const PT uui = g2.x, alphai = g2.y, alphaui = g2.z, alphaBi = g2.w;
-    const PT hi = g4.x;
+    // h_i enters only the FP64 time-step bound
+    const PT hi = dust ? g4.x : sizeof(PT) == 8 ? (PT)a.f[F_H][i] : PT(0);
     const PT hi1p = (PT)hi1;
     const PT hfacgrkerni = pown_t<NDIM>(hi1p) * hi1p;
     PT Bi[3] = {0, 0, 0}, psii = 0;
-    if (mhd) { const T4 g3 = G3[k]; Bi[0] = g3.x; Bi[1] = g3.y; Bi[2] = g3.z; psii = g3.w; }
+    if (mhd) { const T4 g3 = ri[3]; Bi[0] = g3.x; Bi[1] = g3.y; Bi[2] = g3.z; psii = g3.w; }
     const PT Brhoi[3] = {Bi[0] * rho1i, Bi[1] * rho1i, Bi[2] * rho1i};
     const PT B2i = Bi[0] * Bi[0] + Bi[1] * Bi[1] + Bi[2] * Bi[2];
     const PT Brho2i = B2i * rho21i;
     const PT valfven2i = B2i * rho1i;
     const PT epsi = dust ? g4.z : PT(0);
     const PT rhogasi = dust ? g4.w : rhoi;
-    const PT rhodusti = dust ? G5[k] : PT(0);
+    const PT rhodusti = dust ? ri[4].x : PT(0);
     const PT stressmax = (PT)a.sc->stressmax;
     const PT tstop = (PT)p.Kdrag;
     const double rad = p.radkern * a.sc->hmax_nr;
@@ -1762,15 +1770,16 @@ __device__ __forceinline__ void b_force(KArgs a, Phys p, const Grid* __restrict_
     auto loadj = [&](int kk) {
       FRec r;
       r.p = a.P4[kk];
-      r.g0 = G0[kk];
-      r.g1 = G1[kk];
-      r.g2 = G2[kk];
+      const T4* rj = FR + (size_t)kk * NS;
+      r.g0 = rj[0];
+      r.g1 = rj[1];
+      r.g2 = rj[2];
       // (h, EOS density) of a gas without dust are 1/P4.w and rho: no load
-      if (dust) r.g4 = G4[kk];
+      if (dust) r.g4 = rj[3];
       else r.g4 = T4{PT(0), PT(0), PT(0), PT(0)};
-      if (mhd) r.g3 = G3[kk];
+      if (mhd) r.g3 = rj[3];
       else r.g3 = T4{PT(0), PT(0), PT(0), PT(0)};
-      r.g5 = dust ? G5[kk] : PT(0);
+      r.g5 = dust ? rj[4].x : PT(0);
       return r;
     };
     auto body = [&](int kk, const FRec& rj, const double* xj, const int* flip, int code) {
@@ -3470,12 +3479,7 @@ int ndsp_create(const int32_t* iopt, const double* dopt, int64_t npart, int32_t 
   const size_t v4 = sizeof(double4) * n, v1 = sizeof(double) * n;
   ok &= hipMalloc(&a.D4, v4) == hipSuccess;
   ok &= hipMalloc(&a.De, v1) == hipSuccess;
-  ok &= hipMalloc(&a.G0, v4) == hipSuccess;
-  ok &= hipMalloc(&a.G1, v4) == hipSuccess;
-  ok &= hipMalloc(&a.G2, v4) == hipSuccess;
-  ok &= hipMalloc(&a.G3, v4) == hipSuccess;
-  ok &= hipMalloc(&a.G4, v4) == hipSuccess;
-  ok &= hipMalloc(&a.G5, v1) == hipSuccess;
+  ok &= hipMalloc(&a.FR, v4 * fr_slots(fr_phys(c->p))) == hipSuccess;
   if (c->p.fixed_on) ok &= hipMalloc(&a.GF, v4) == hipSuccess;
   ok &= hipMalloc((void**)&a.sc, sizeof(DScal)) == hipSuccess;
   a.rcap = (int)nblk(n, 64);
@@ -3529,7 +3533,7 @@ int ndsp_destroy(ndsp_ctx* c) {
   KArgs& a = c->a;
   for (int fi = 0; fi < F_NFIELDS; fi++) if (a.f[fi]) hipFree(a.f[fi]);
   void* ptrs[] = {c->scratch_d, c->scratch_i, a.numneigh, a.nits, a.uid, a.itype, a.ireal, a.slot_of, a.cellid, a.sidx, a.P4, a.dxb,
-                  a.D4, a.De, a.G0, a.G1, a.G2, a.G3, a.G4, a.G5, a.GF, a.sc, (void*)a.tabf, (void*)a.tabd,
+                  a.D4, a.De, a.FR, a.GF, a.sc, (void*)a.tabf, (void*)a.tabd,
                   c->part, c->cons_out, a.cstart, a.cfill, c->bsum, c->list_a, c->list_b, a.nrflag, c->cmp_bsum,
                   a.nbl, a.nbc, a.nbx, a.hcap, a.red, c->gdev};
   for (void* q : ptrs) if (q) hipFree(q);

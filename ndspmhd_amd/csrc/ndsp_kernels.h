@@ -112,12 +112,10 @@ struct KArgs {
   double* dxb;        // reflecting ghost extent radkern*h at ghost-creation time
   void* D4;           // density: vx, vy, vz, m
   void* De;           // density: dust fraction
-  void* G0;           // force: vx vy vz m
-  void* G1;           // force: rho pr cs gradh
-  void* G2;           // force: uu a0 a1 a2
-  void* G3;           // force: Bx By Bz psi
-  void* G4;           // force: h dens eps rhogas
-  void* G5;           // force: rhodust
+  // force: one record of quads (4 reals) per sorted particle, fr_slots()
+  // quads long: (vx vy vz m) (rho pr cs gradh) (uu a0 a1 a2), then
+  // (Bx By Bz psi) with MHD or (h dens eps rhogas) (rhodust 0 0 0) with dust
+  void* FR;
   void* GF;           // force, images only: m a0 a1 a2 (fixed particles: before the copy)
   // kernel tables (FP32 and FP64 copies of W, dW, d2W)
   const float* tabf;  // 3 x (IKERN+1)
