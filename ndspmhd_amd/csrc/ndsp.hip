@@ -501,19 +501,45 @@ __global__ __launch_bounds__(BS) void k_cellsort_wave(KArgs a, const Grid* __res
 // neighbour views
 // ===========================================================================
 
+// bytes of a density record: the position and 1/h (double4), then one quad
+// of reals, two with dust, padded to 32 bytes
+template <typename PT>
+__host__ __device__ constexpr int dr_bytes(bool dust) {
+  return (32 + (dust ? 2 : 1) * 4 * (int)sizeof(PT) + 31) / 32 * 32;
+}
+
+// The pack kernels form each particle's record in LDS, then the workgroup
+// writes its records (one contiguous run) with consecutive 16-byte stores:
+// records written in place by their own lanes would be strided stores.
+__device__ __forceinline__ void flush_records(char* __restrict__ dst, const float4* __restrict__ stage, int cnt,
+                                              int rb) {
+  const int nq = cnt * rb / 16;
+  float4* d = (float4*)dst;
+  for (int q = threadIdx.x; q < nq; q += blockDim.x) d[q] = stage[q];
+}
+
 template <typename PT>
 __device__ __forceinline__ void b_pack_density(KArgs a, Phys p, VL vl) {
-  int k = vl.b * blockDim.x + threadIdx.x;
-  if (k >= a.n) return;
-  int i = a.sidx[k];
-  double h = a.f[F_H][i];
-  a.P4[k] = make_double4(a.f[F_X][i], a.f[F_Y][i], a.f[F_Z][i], 1.0 / h);
-  a.dxb[k] = p.radkern * h;
-  typedef typename V4<PT>::type T4;
-  ((T4*)a.D4)[k] = T4{(PT)a.f[F_VX][i], (PT)a.f[F_VY][i], (PT)a.f[F_VZ][i], (PT)a.f[F_M][i]};
-  // the density sums read the dust fraction of the last conservative2primitive
-  // (dustfrac, src/density_sums.f90:276-278), not the predicted dustevol
-  if (p.idust) ((PT*)a.De)[k] = (PT)a.f[F_DUSTFRAC][i];
+  alignas(32) __shared__ float4 stage[BS * dr_bytes<PT>(true) / 16];
+  const int rb = dr_bytes<PT>(p.idust != 0);
+  const int k0 = vl.b * blockDim.x, k = k0 + threadIdx.x;
+  if (k < a.n) {
+    int i = a.sidx[k];
+    double h = a.f[F_H][i];
+    const double4 pk = make_double4(a.f[F_X][i], a.f[F_Y][i], a.f[F_Z][i], 1.0 / h);
+    a.P4[k] = pk;
+    a.dxb[k] = p.radkern * h;
+    typedef typename V4<PT>::type T4;
+    char* rec = (char*)stage + threadIdx.x * rb;
+    *(double4*)rec = pk;
+    T4* R = (T4*)(rec + 32);
+    R[0] = T4{(PT)a.f[F_VX][i], (PT)a.f[F_VY][i], (PT)a.f[F_VZ][i], (PT)a.f[F_M][i]};
+    // the density sums read the dust fraction of the last conservative2primitive
+    // (dustfrac, src/density_sums.f90:276-278), not the predicted dustevol
+    if (p.idust) R[1] = T4{(PT)a.f[F_DUSTFRAC][i], PT(0), PT(0), PT(0)};
+  }
+  __syncthreads();
+  flush_records((char*)a.DR + (size_t)k0 * rb, stage, min((int)blockDim.x, a.n - k0), rb);
 }
 template <typename PT>
 __global__ void k_pack_density(KArgs a, Phys p) {
@@ -524,19 +550,28 @@ __global__ void k_pack_density(KArgs a, Phys p) {
 // else dust, else hydro) fixes the layout
 __host__ __device__ __forceinline__ int fr_phys(const Phys& p) { return p.imhd ? 1 : p.idust ? 2 : 0; }
 __host__ __device__ constexpr int fr_slots(int phys) { return phys == 1 ? 4 : phys == 2 ? 5 : 3; }
-
+// bytes of a record: the position and 1/h (double4), then the quads, padded
+// to 32 bytes
 template <typename PT>
-__device__ __forceinline__ void b_pack_force(KArgs a, Phys p, VL vl) {
-  int k = vl.b * blockDim.x + threadIdx.x;
-  if (k >= a.n) return;
+__host__ __device__ constexpr size_t fr_bytes(int phys) {
+  return (32 + (size_t)fr_slots(phys) * 4 * sizeof(PT) + 31) / 32 * 32;
+}
+
+// the force record of sorted particle k into rec, and the copies the force
+// pass leaves behind
+template <typename PT>
+__device__ __forceinline__ void pack_force_one(KArgs& a, const Phys& p, int k, char* rec) {
   int i = a.sidx[k];
   double h = a.f[F_H][i];
-  a.P4[k].w = 1.0 / h;
+  double4 pk = a.P4[k];
+  pk.w = 1.0 / h;
+  a.P4[k].w = pk.w;
   // a fixed particle's h was replaced by its partner's after the iteration
   if (p.fixed_on && p.use_list && a.itype[i] == 1 && h > a.hcap[k]) atomicOr(&a.sc->list_invalid, 1);
   typedef typename V4<PT>::type T4;
   const int phys = fr_phys(p);
-  T4* R = (T4*)a.FR + (size_t)k * fr_slots(phys);
+  *(double4*)rec = pk;
+  T4* R = (T4*)(rec + 32);
   R[0] = T4{(PT)a.f[F_VX][i], (PT)a.f[F_VY][i], (PT)a.f[F_VZ][i], (PT)a.f[F_M][i]};
   R[1] = T4{(PT)a.f[F_RHO][i], (PT)a.f[F_PR][i], (PT)a.f[F_CS][i], (PT)a.f[F_GRADH][i]};
   R[2] = T4{(PT)a.f[F_UU][i], (PT)a.f[F_A0][i], (PT)a.f[F_A1][i], (PT)a.f[F_A2][i]};
@@ -556,6 +591,17 @@ __device__ __forceinline__ void b_pack_force(KArgs a, Phys p, VL vl) {
                         (PT)a.f[fx ? F_A1IMG : F_A1][i], (PT)a.f[fx ? F_A2IMG : F_A2][i]};
   }
 }
+template <typename PT>
+__device__ __forceinline__ void b_pack_force(KArgs a, Phys p, VL vl) {
+  alignas(32) __shared__ float4 stage[BS * fr_bytes<PT>(2) / 16];
+  const int phys = fr_phys(p);
+  const int rb = (int)fr_bytes<PT>(phys);
+  const int k0 = vl.b * blockDim.x, k = k0 + threadIdx.x;
+  if (k < a.n) pack_force_one<PT>(a, p, k, (char*)stage + threadIdx.x * rb);
+  __syncthreads();
+  flush_records((char*)a.FR + (size_t)k0 * rb, stage, min((int)blockDim.x, a.n - k0), rb);
+}
+
 template <typename PT>
 __global__ void k_pack_force(KArgs a, Phys p) {
   b_pack_force<PT>(a, p, VL{(int)blockIdx.x, (int)gridDim.x});
@@ -1327,8 +1373,8 @@ __device__ __forceinline__ void b_nr_pass(KArgs a, Phys p, const Grid* __restric
   bool unconverged = false;
   const int k = active ? (list ? list[tid] : tid) : 0;
   typedef typename V4<PT>::type T4;
-  const T4* D4 = (const T4*)a.D4;
-  const PT* De = (const PT*)a.De;
+  constexpr int RB = dr_bytes<PT>(DUST);
+  const char* DR = (const char*)a.DR;
   // this thread's share of the global reductions
   double r_stress = 0.0, r_hmax = 0.0, r_its = 0.0, r_nmin = DBL_MAX, r_nmax = 0.0, r_pairs = 0.0, r_ncalc = 0.0;
   // fixed particles are not iterated (src/iterate_density.f90:166,253) and
@@ -1344,7 +1390,7 @@ __device__ __forceinline__ void b_nr_pass(KArgs a, Phys p, const Grid* __restric
     if (its0 == 1) { hin = h; a.f[F_HNR0][i] = h; }
     else hin = a.f[F_HNR0][i];
     const double mi = a.f[F_M][i];
-    const T4 dk = D4[k];
+    const T4 dk = *(const T4*)(DR + (size_t)k * RB + 32);
     const PT vi[3] = {dk.x, dk.y, dk.z};
     const PT ddq2 = (PT)p.ddq2table, dq2 = (PT)p.dq2table;
     const double R2 = p.radkern2;
@@ -1357,9 +1403,11 @@ __device__ __forceinline__ void b_nr_pass(KArgs a, Phys p, const Grid* __restric
     };
     auto loadj = [&](int kk) {
       DRec r;
-      r.p = a.P4[kk];
-      r.d = D4[kk];
-      r.e = dust ? De[kk] : PT(0);
+      const char* rec = DR + (size_t)kk * RB;
+      r.p = *(const double4*)rec;
+      const T4* q = (const T4*)(rec + 32);
+      r.d = q[0];
+      r.e = dust ? q[1].x : PT(0);
       return r;
     };
 
@@ -1554,6 +1602,7 @@ __device__ __forceinline__ void b_refresh_hinv(KArgs a, Phys p, VL vl) {
   if (k >= a.n) return;
   const double h = a.f[F_H][a.sidx[k]];
   a.P4[k].w = 1.0 / h;
+  ((double4*)((char*)a.DR + (size_t)k * a.dr_stride))->w = 1.0 / h;
   if (refresh_dxb) a.dxb[k] = p.radkern * h;
 }
 __global__ void k_refresh_hinv(KArgs a, Phys p) {
@@ -1708,8 +1757,8 @@ __device__ __forceinline__ void b_force(KArgs a, Phys p, const Grid* __restrict_
   typedef typename V4<PT>::type T4;
   constexpr bool mhd = PHYS == 1;
   constexpr bool dust = PHYS == 2;
-  constexpr int NS = fr_slots(PHYS);
-  const T4* FR = (const T4*)a.FR;
+  constexpr size_t RB = fr_bytes<PT>(PHYS);
+  const char* FR = (const char*)a.FR;
   const T4* GF = (const T4*)a.GF;
   const PT ddq2 = (PT)p.ddq2table, dq2 = (PT)p.dq2table;
   const double R2 = p.radkern2;
@@ -1724,11 +1773,12 @@ __device__ __forceinline__ void b_force(KArgs a, Phys p, const Grid* __restrict_
   int err = 0;
   if (active) {
     const int i = a.sidx[k];
-    const double4 pk = a.P4[k];
+    const char* rec = FR + (size_t)k * RB;
+    const double4 pk = *(const double4*)rec;
     const double xi[3] = {pk.x, pk.y, pk.z};
     const double hi1 = pk.w;
     const double hi21 = mul_exact(hi1, hi1);
-    const T4* ri = FR + (size_t)k * NS;
+    const T4* ri = (const T4*)(rec + 32);
     const T4 g0 = ri[0], g1 = ri[1], g2 = ri[2];
     const T4 g4 = dust ? ri[3] : T4{PT(0), PT(0), PT(0), PT(0)};
     const PT vi[3] = {g0.x, g0.y, g0.z};
@@ -1769,8 +1819,9 @@ __device__ __forceinline__ void b_force(KArgs a, Phys p, const Grid* __restrict_
     };
     auto loadj = [&](int kk) {
       FRec r;
-      r.p = a.P4[kk];
-      const T4* rj = FR + (size_t)kk * NS;
+      const char* rec = FR + (size_t)kk * RB;
+      r.p = *(const double4*)rec;
+      const T4* rj = (const T4*)(rec + 32);
       r.g0 = rj[0];
       r.g1 = rj[1];
       r.g2 = rj[2];
@@ -3477,9 +3528,9 @@ int ndsp_create(const int32_t* iopt, const double* dopt, int64_t npart, int32_t 
   ok &= hipMalloc(&a.P4, sizeof(double4) * n) == hipSuccess;
   ok &= hipMalloc(&a.dxb, sizeof(double) * n) == hipSuccess;
   const size_t v4 = sizeof(double4) * n, v1 = sizeof(double) * n;
-  ok &= hipMalloc(&a.D4, v4) == hipSuccess;
-  ok &= hipMalloc(&a.De, v1) == hipSuccess;
-  ok &= hipMalloc(&a.FR, v4 * fr_slots(fr_phys(c->p))) == hipSuccess;
+  a.dr_stride = c->prec == NDSP_PREC_FP64 ? dr_bytes<double>(c->p.idust != 0) : dr_bytes<float>(c->p.idust != 0);
+  ok &= hipMalloc(&a.DR, (size_t)a.dr_stride * n) == hipSuccess;
+  ok &= hipMalloc(&a.FR, fr_bytes<double>(fr_phys(c->p)) * n) == hipSuccess;
   if (c->p.fixed_on) ok &= hipMalloc(&a.GF, v4) == hipSuccess;
   ok &= hipMalloc((void**)&a.sc, sizeof(DScal)) == hipSuccess;
   a.rcap = (int)nblk(n, 64);
@@ -3533,7 +3584,7 @@ int ndsp_destroy(ndsp_ctx* c) {
   KArgs& a = c->a;
   for (int fi = 0; fi < F_NFIELDS; fi++) if (a.f[fi]) hipFree(a.f[fi]);
   void* ptrs[] = {c->scratch_d, c->scratch_i, a.numneigh, a.nits, a.uid, a.itype, a.ireal, a.slot_of, a.cellid, a.sidx, a.P4, a.dxb,
-                  a.D4, a.De, a.FR, a.GF, a.sc, (void*)a.tabf, (void*)a.tabd,
+                  a.DR, a.FR, a.GF, a.sc, (void*)a.tabf, (void*)a.tabd,
                   c->part, c->cons_out, a.cstart, a.cfill, c->bsum, c->list_a, c->list_b, a.nrflag, c->cmp_bsum,
                   a.nbl, a.nbc, a.nbx, a.hcap, a.red, c->gdev};
   for (void* q : ptrs) if (q) hipFree(q);

@@ -110,11 +110,14 @@ struct KArgs {
   // packed neighbour views in sorted order
   double4* P4;        // x, y, z, 1/h
   double* dxb;        // reflecting ghost extent radkern*h at ghost-creation time
-  void* D4;           // density: vx, vy, vz, m
-  void* De;           // density: dust fraction
-  // force: one record of quads (4 reals) per sorted particle, fr_slots()
-  // quads long: (vx vy vz m) (rho pr cs gradh) (uu a0 a1 a2), then
-  // (Bx By Bz psi) with MHD or (h dens eps rhogas) (rhodust 0 0 0) with dust
+  // density: one record per sorted particle (dr_stride bytes): x y z 1/h
+  // in FP64, then the reals (vx vy vz m) and, with dust, (eps 0 0 0)
+  void* DR;
+  int dr_stride;
+  // force: one record per sorted particle (fr_bytes() long): x y z 1/h in
+  // FP64, then quads of reals (vx vy vz m) (rho pr cs gradh) (uu a0 a1 a2)
+  // and (Bx By Bz psi) with MHD or (h dens eps rhogas) (rhodust 0 0 0) with
+  // dust
   void* FR;
   void* GF;           // force, images only: m a0 a1 a2 (fixed particles: before the copy)
   // kernel tables (FP32 and FP64 copies of W, dW, d2W)
