@@ -502,10 +502,11 @@ __global__ __launch_bounds__(BS) void k_cellsort_wave(KArgs a, const Grid* __res
 // ===========================================================================
 
 // bytes of a density record: the position and 1/h (double4), then one quad
-// of reals, two with dust, padded to 32 bytes
+// of reals, two with dust (48 bytes in FP32: records straddle cache lines,
+// and the pass runs faster on fewer bytes than on 64-byte aligned records)
 template <typename PT>
 __host__ __device__ constexpr int dr_bytes(bool dust) {
-  return (32 + (dust ? 2 : 1) * 4 * (int)sizeof(PT) + 31) / 32 * 32;
+  return (32 + (dust ? 2 : 1) * 4 * (int)sizeof(PT) + 15) / 16 * 16;
 }
 
 // The pack kernels form each particle's record in LDS, then the workgroup
@@ -2386,6 +2387,14 @@ __device__ __forceinline__ void b_correct(KArgs a, Phys p, VL vl) {
 __global__ void k_correct(KArgs a, Phys p) {
   b_correct(a, p, VL{(int)blockIdx.x, (int)gridDim.x});
 }
+// finalize, then the corrector, particle by particle: neither reads another
+// particle's fields, and the rates the corrector reads were just written by
+// the same thread (no second pass over them from HBM)
+__global__ void k_finalize_correct(KArgs a, Phys p) {
+  const VL vl{(int)blockIdx.x, (int)gridDim.x};
+  b_finalize(a, p, vl);
+  b_correct(a, p, vl);
+}
 
 // ===========================================================================
 // FP64 conserved-quantity reduction, src/evwrite_mhd.f90:124-248
@@ -3272,7 +3281,9 @@ static int density_finish(ndsp_ctx* c) {
   return set_err(NDSP_ERR_NOT_CONVERGED, "h-rho iteration did not finish");
 }
 
-static int do_rates(ndsp_ctx* c) {
+// with_correct: the step's corrector runs in the finalize launch (each
+// particle's final rates, then its corrected state)
+static int do_rates(ndsp_ctx* c, bool with_correct = false) {
   KArgs& a = c->a;
   const int n = c->n;
   if (c->p.fixed_on) hipLaunchKernelGGL(k_fixed_copy, dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p);
@@ -3308,7 +3319,8 @@ static int do_rates(ndsp_ctx* c) {
   }
   {
     TScope t(c, "finalize");
-    hipLaunchKernelGGL(k_finalize, dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p);
+    if (with_correct) hipLaunchKernelGGL(k_finalize_correct, dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p);
+    else hipLaunchKernelGGL(k_finalize, dim3(nblk(n)), dim3(BS), 0, c->st, a, c->p);
     hipLaunchKernelGGL(k_red_final, dim3(1), dim3(BS), 0, c->st, (const double*)a.red, a.rcap, (int)nblk(n),
                        red_spec({RED_Q(RED_MAX, RED_F64, fhmax), RED_Q(RED_MIN, RED_F64, dtdrag)}), a.sc,
                        (const int*)nullptr);
@@ -3827,10 +3839,9 @@ static int step_part_b(ndsp_ctx* c) {
     TScope t(c, "mlp_target");
     hipLaunchKernelGGL(k_mlp_target, dim3(nblk(c->n)), dim3(BS), 0, c->st, c->a, ml.X);
   }
-  if (int e = do_rates(c)) return e;
+  if (int e = do_rates(c, true)) return e;
   {
     TScope t(c, "correct");
-    hipLaunchKernelGGL(k_correct, dim3(nblk(c->n)), dim3(BS), 0, c->st, c->a, c->p);
     hipLaunchKernelGGL(k_boundary, dim3(nblk(c->n)), dim3(BS), 0, c->st, c->a, c->p, c->g, 1);
     hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, c->st, c->a.sc);
   }
