@@ -18,7 +18,8 @@
 //                   1175-1690, 1700-1894, 2154-2206, 2377-2720, 2813-2878)
 //   k_finalize      per-particle finalisation (src/ratesND_mhd.f90:482-943)
 //   k_dt            timestep (src/stepND_leapfrog_mhd.f90:252)
-// and of one step: k_predict, derivs, k_correct (stepND_leapfrog_mhd.f90:31-275).
+// and of one step: k_predict, derivs with the corrector in the finalize launch
+// (k_finalize_correct) (stepND_leapfrog_mhd.f90:31-275).
 //
 // Periodic and reflecting boundaries use image cells instead of ghost copies;
 // the image coordinates reproduce the ghost arithmetic of
@@ -2383,9 +2384,6 @@ __device__ __forceinline__ void b_correct(KArgs a, Phys p, VL vl) {
     if (p.iavlim[c] != 0) f[as[c]][i] = fmin(f[F_A0IN + c][i] + hdt * (f[das[c]][i] + f[F_DA0IN + c][i]), 1.0);
   if (p.idivbzero >= 2) f[F_PSI][i] = f[F_PSIIN][i] + hdt * (f[F_DPSIDT][i] + f[F_DPSIDTIN][i]);
   if (p.idust) f[F_EPS][i] = f[F_EPSIN][i] + hdt * (f[F_DEPS][i] + f[F_DEPSIN][i]);
-}
-__global__ void k_correct(KArgs a, Phys p) {
-  b_correct(a, p, VL{(int)blockIdx.x, (int)gridDim.x});
 }
 // finalize, then the corrector, particle by particle: neither reads another
 // particle's fields, and the rates the corrector reads were just written by
