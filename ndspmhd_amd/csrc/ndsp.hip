@@ -692,9 +692,10 @@ __device__ __forceinline__ unsigned inverse_code(unsigned code) {
 // image coordinates of the ghost arithmetic (so the exact FP64 tests see the
 // values of the cell scan).  Entries come four per load; the record of entry
 // m + PF (PF 1 or 2) is fetched while entry m is processed, so a wavefront's
-// gathers overlap its arithmetic (two ahead where the pair arithmetic is
-// light and the records small).  The loop is unrolled over the PF + 1 record
-// buffers, which take turns as the current one, so no record is copied.
+// gathers overlap its arithmetic (the pair passes use one ahead: a second
+// record in flight costs them a wave per SIMD).  The loop is unrolled over
+// the PF + 1 record buffers, which take turns as the current one, so no
+// record is copied.
 template <int NDIM, int PF = 1, typename LOAD, typename F>
 __device__ __forceinline__ void for_each_listed_pf(const KArgs& a, const Grid& g, int k, int cnt, LOAD&& load,
                                                    F&& body) {
@@ -1476,7 +1477,7 @@ __device__ __forceinline__ void b_nr_pass(KArgs a, Phys p, const Grid* __restric
           drhodt.add(mj * dvdotr * grkerni);
         }
       };
-      if (uselist) for_each_listed_pf<NDIM, 2>(a, g, k, nlisted, loadj, body);
+      if (uselist) for_each_listed_pf<NDIM, 1>(a, g, k, nlisted, loadj, body);
       else for_each_candidate_rec<NDIM>(a, g, xi, rad, loadj, body);
       npairs += (unsigned long long)ngat;
       nn = symp ? nsym : ngat;
@@ -1589,9 +1590,11 @@ __device__ __forceinline__ void b_nr_pass(KArgs a, Phys p, const Grid* __restric
     if (unconverged) next_list[base + __popcll(m & ((1ull << lane) - 1ull))] = k;
   }
 }
+// three waves per SIMD (an occupancy attribute): with one record in flight
+// the FP32 pass fits in 168 registers, spilling two outside the pair loop
 template <int NDIM, typename PT, bool DUST, int TB>
-__global__ __launch_bounds__(TB) void k_nr_pass(KArgs a, Phys p, const Grid* __restrict__ gp, int* __restrict__ list_a,
-                                                int* __restrict__ list_b) {
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(3))) void k_nr_pass(
+    KArgs a, Phys p, const Grid* __restrict__ gp, int* __restrict__ list_a, int* __restrict__ list_b) {
   b_nr_pass<NDIM, PT, DUST, TB>(a, p, gp, list_a, list_b, VL{(int)blockIdx.x, (int)gridDim.x});
 }
 
@@ -2029,9 +2032,9 @@ __device__ __forceinline__ void b_force(KArgs a, Phys p, const Grid* __restrict_
       }
     };
     if (!err) {
-      // two records ahead for FP32 records; an FP64 force record is twice the
-      // registers and one ahead keeps more waves resident
-      if (uselist) for_each_listed_pf<NDIM, sizeof(PT) == 4 ? 2 : 1>(a, g, k, a.nbc[k] + a.nbx[k], loadj, body);
+      // one record ahead: a second 96-byte record in flight would cost the
+      // third wave per SIMD (162 registers with one, 180 with two)
+      if (uselist) for_each_listed_pf<NDIM, 1>(a, g, k, a.nbc[k] + a.nbx[k], loadj, body);
       else for_each_candidate_rec<NDIM>(a, g, xi, rad, loadj, body);
     }
     st_stream(&a.f[F_FX][i], fx.value());
