@@ -401,8 +401,8 @@ __global__ void k_scatter(KArgs a) {
 
 // ascending order of N values held in registers (bitonic network; every
 // index is fixed at compile time, so v stays in registers)
-template <int N>
-__device__ __forceinline__ void sort_regs(int (&v)[N]) {
+template <int N, typename T>
+__device__ __forceinline__ void sort_regs(T (&v)[N]) {
 #pragma unroll
   for (int k = 2; k <= N; k <<= 1)
 #pragma unroll
@@ -411,7 +411,7 @@ __device__ __forceinline__ void sort_regs(int (&v)[N]) {
       for (int i = 0; i < N; i++) {
         const int l = i ^ j;
         if (l > i) {
-          const int x = v[i], y = v[l];
+          const T x = v[i], y = v[l];
           const bool up = (i & k) == 0;
           v[i] = up ? min(x, y) : max(x, y);
           v[l] = up ? max(x, y) : min(x, y);
@@ -1139,6 +1139,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   b_nblist<NDIM>(a, p, gp, VL{(int)blockIdx.x, (int)gridDim.x});
 }
 
+// nx <= N entries E[0..nx) sorted ascending into D[0..nx) (D may overlap E:
+// every entry is read before any is written)
+template <int N>
+__device__ __forceinline__ void sort_extras(const unsigned* E, unsigned* D, int nx) {
+  unsigned v[N];
+#pragma unroll
+  for (int m = 0; m < N; m++) v[m] = m < nx ? E[m] : 0xffffffffu;
+  sort_regs<N>(v);
+#pragma unroll
+  for (int m = 0; m < N; m++)
+    if (m < nx) D[m] = v[m];
+}
+
 // extras (written from the end of the row) into [nbc, nbc + nbx) in
 // ascending entry order, so sums are deterministic; a row that did not hold
 // the gather list and the extras falls back to the cell scan
@@ -1154,25 +1167,14 @@ __device__ __forceinline__ void b_nblist_sort(KArgs a, VL vl) {
   // extras occupy E = L[maxnb-nx .. maxnb-1]; they end at L[c0 ..] ascending
   unsigned* E = a.nbl + (size_t)k * a.maxnb + (a.maxnb - nx);
   unsigned* D = a.nbl + (size_t)k * a.maxnb + c0;
-  constexpr int SMAX = 32;
-  if (nx <= SMAX) {
-    // up to 32 of them sorted in registers (odd-even transposition, every
-    // index a constant), then stored in place
-    unsigned v[SMAX];
-#pragma unroll
-    for (int m = 0; m < SMAX; m++) v[m] = m < nx ? E[m] : 0xffffffffu;
-#pragma unroll
-    for (int r = 0; r < SMAX; r++) {
-#pragma unroll
-      for (int m = r & 1; m + 1 < SMAX; m += 2) {
-        const unsigned lo = min(v[m], v[m + 1]), hi = max(v[m], v[m + 1]);
-        v[m] = lo;
-        v[m + 1] = hi;
-      }
-    }
-#pragma unroll
-    for (int m = 0; m < SMAX; m++)
-      if (m < nx) D[m] = v[m];
+  if (nx <= 32) {
+    // up to 32 of them sorted in registers (a bitonic network of 8, 16 or
+    // 32, the smallest that holds every such row of the wave; every index a
+    // constant), then stored in place
+    const bool w16 = __ballot(nx > 16) != 0, w8 = __ballot(nx > 8) != 0;
+    if (w16) sort_extras<32>(E, D, nx);
+    else if (w8) sort_extras<16>(E, D, nx);
+    else sort_extras<8>(E, D, nx);
     return;
   }
   // more: insertion sort in place, then move down (ascending copy,
