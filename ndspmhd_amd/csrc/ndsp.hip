@@ -76,14 +76,14 @@ __device__ __forceinline__ void b_boundary(KArgs a, Phys p, Grid g, int in_step,
   double* xins[3] = {a.f[F_XIN], a.f[F_YIN], a.f[F_ZIN]};
   double* vins[3] = {a.f[F_VXIN], a.f[F_VYIN], a.f[F_VZIN]};
   double* fs[3] = {a.f[F_FX], a.f[F_FY], a.f[F_FZ]};
-  for (int d = 0; d < p.ndim; d++) {
+  for (int d = 0; d < 3; d++) if (d < p.ndim) {
     if (g.bound[d] != 3) continue;
     double x = xs[d][i];
     if (x > g.xmax[d]) x = g.xmin[d] + x - g.xmax[d];
     else if (x < g.xmin[d]) x = g.xmax[d] - (g.xmin[d] - x);
     xs[d][i] = x;
   }
-  for (int d = 0; d < p.ndim; d++) {
+  for (int d = 0; d < 3; d++) if (d < p.ndim) {
     if (g.bound[d] <= 1) continue;
     double x = xs[d][i];
     bool hit = false;
@@ -2278,7 +2278,7 @@ __device__ __forceinline__ void b_predict(KArgs a, Phys p, VL vl) {
   f[F_DENDTIN][i] = dendt;
   f[F_DPSIDTIN][i] = dpsidt;
   f[F_DEPSIN][i] = deps;
-  for (int d = 0; d < p.ndim; d++) f[xs[d]][i] = x[d] + dt * v[d] + 0.5 * dt * dt * fo[d];
+  for (int d = 0; d < 3; d++) if (d < p.ndim) f[xs[d]][i] = x[d] + dt * v[d] + 0.5 * dt * dt * fo[d];
   for (int c = 0; c < 3; c++) f[vs[c]][i] = (v[c] + dt * fo[c]) / 1.0;
   if (p.imhd != 0)
     for (int c = 0; c < 3; c++) f[bs[c]][i] = b[c] + dt * db[c];
@@ -2306,7 +2306,7 @@ __device__ __forceinline__ void b_predict_fixed(KArgs a, Phys p, VL vl) {
   const int j = a.slot_of[a.ireal[i]];
   static const int xs[3] = {F_X, F_Y, F_Z}, vs[3] = {F_VX, F_VY, F_VZ}, bs[3] = {F_BX, F_BY, F_BZ};
   static const int as[3] = {F_A0, F_A1, F_A2};
-  for (int d = 0; d < p.ndim; d++)
+  for (int d = 0; d < 3; d++) if (d < p.ndim)
     f[xs[d]][i] = f[F_XIN + d][i] + dt * f[F_VXIN + d][j] + 0.5 * dt * dt * f[F_FXIN + d][j];
   for (int c = 0; c < 3; c++) {
     f[vs[c]][i] = f[F_VXIN + c][i];
