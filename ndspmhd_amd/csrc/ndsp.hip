@@ -2709,6 +2709,20 @@ __global__ void k_gather_set(FieldSet<T> fs, T* __restrict__ scratch, const int*
   for (int q = 0; q < REORDER_W; q++)
     if (q < fs.nf) scratch[(size_t)q * n + k] = fs.f[q][i];
 }
+// src[q][sidx[k]] -> dst[q][k]: a reorder into the other field set
+struct FieldPairs {
+  const double* src[REORDER_W];
+  double* dst[REORDER_W];
+  int nf;
+};
+__global__ void k_gather_pairs(FieldPairs fp, const int* __restrict__ sidx, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int i = sidx[k];
+#pragma unroll
+  for (int q = 0; q < REORDER_W; q++)
+    if (q < fp.nf) fp.dst[q][k] = fp.src[q][i];
+}
 template <typename T>
 __global__ void k_copy_set(FieldSet<T> fs, const T* __restrict__ scratch, int n) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2991,6 +3005,15 @@ struct ndsp_ctx {
   bool use_graph = false;   // ndsp_step replays recorded graphs
   bool graph_valid = false;
   hipGraphExec_t graph_a = nullptr, graph_b = nullptr;
+  // second set of the persistent fields: a reorder gathers into it and the
+  // two sets trade places, each with its own recorded halves of a step
+  // (members of a batch, whose argument tables hold the field addresses,
+  // copy back instead)
+  double* falt[F_NPERSIST] = {};
+  bool falt_failed = false;
+  bool in_batch = false;
+  bool alt_graph_valid = false;
+  hipGraphExec_t alt_graph_a = nullptr, alt_graph_b = nullptr;
   Phys p;
   int prec;
   int n;
@@ -3026,6 +3049,12 @@ struct ndsp_ctx {
   std::vector<std::pair<int, int>> pending;  // (label, event pair index)
   int nev = 0;
 };
+
+// recorded steps hold field addresses and launch shapes: both sets' go stale
+static void invalidate_graphs(ndsp_ctx* c) {
+  c->graph_valid = false;
+  c->alt_graph_valid = false;
+}
 
 // ---- timing helpers ------------------------------------------------------
 static int label_id(ndsp_ctx* c, const char* name) {
@@ -3125,7 +3154,7 @@ static int size_key_space(ndsp_ctx* c, const DScal& hs, int extra_bits) {
   HIPCHK(hipMalloc(&c->a.cfill, sizeof(int) * cap));
   HIPCHK(hipMalloc(&c->bsum, sizeof(int) * (cap / (BS * 4) + 2)));
   c->cap_bits = bits;
-  c->graph_valid = false;
+  invalidate_graphs(c);
   return 0;
 }
 
@@ -3147,7 +3176,7 @@ static int grow_lists(ndsp_ctx* c) {
   HIPCHK(hipFree(c->a.nbl));
   c->a.nbl = nbl;
   c->a.maxnb = m;
-  c->graph_valid = false;
+  invalidate_graphs(c);
   return 0;
 }
 
@@ -3364,7 +3393,36 @@ static int reorder(ndsp_ctx* c) {
   KArgs& a = c->a;
   const int n = c->n;
   TScope t(c, "reorder");
-  for (int f0 = 0; f0 < F_NPERSIST; f0 += REORDER_W) {
+  if (!c->in_batch && !c->falt_failed && !c->falt[0]) {
+    bool ok = true;
+    for (int fi = 0; fi < F_NPERSIST && ok; fi++) ok = hipMalloc(&c->falt[fi], sizeof(double) * n) == hipSuccess;
+    if (!ok) {
+      (void)hipGetLastError();
+      for (int fi = 0; fi < F_NPERSIST; fi++)
+        if (c->falt[fi]) {
+          (void)hipFree(c->falt[fi]);
+          c->falt[fi] = nullptr;
+        }
+      c->falt_failed = true;
+    }
+  }
+  const bool swap_sets = !c->in_batch && c->falt[0] != nullptr;
+  for (int f0 = 0; swap_sets && f0 < F_NPERSIST; f0 += REORDER_W) {
+    FieldPairs fp{};
+    fp.nf = std::min(REORDER_W, (int)F_NPERSIST - f0);
+    for (int q = 0; q < fp.nf; q++) {
+      fp.src[q] = a.f[f0 + q];
+      fp.dst[q] = c->falt[f0 + q];
+    }
+    hipLaunchKernelGGL(k_gather_pairs, dim3(nblk(n)), dim3(BS), 0, c->st, fp, (const int*)a.sidx, n);
+  }
+  if (swap_sets) {
+    for (int fi = 0; fi < F_NPERSIST; fi++) std::swap(a.f[fi], c->falt[fi]);
+    std::swap(c->graph_a, c->alt_graph_a);
+    std::swap(c->graph_b, c->alt_graph_b);
+    std::swap(c->graph_valid, c->alt_graph_valid);
+  }
+  for (int f0 = 0; !swap_sets && f0 < F_NPERSIST; f0 += REORDER_W) {
     FieldSet<double> fs{};
     fs.nf = std::min(REORDER_W, F_NPERSIST - f0);
     for (int q = 0; q < fs.nf; q++) fs.f[q] = a.f[f0 + q];
@@ -3605,6 +3663,9 @@ int ndsp_destroy(ndsp_ctx* c) {
   for (void* q : ptrs) if (q) hipFree(q);
   if (c->graph_a) hipGraphExecDestroy(c->graph_a);
   if (c->graph_b) hipGraphExecDestroy(c->graph_b);
+  if (c->alt_graph_a) hipGraphExecDestroy(c->alt_graph_a);
+  if (c->alt_graph_b) hipGraphExecDestroy(c->alt_graph_b);
+  for (int fi = 0; fi < F_NPERSIST; fi++) if (c->falt[fi]) hipFree(c->falt[fi]);
   if (c->mlp_alloced) mlp_free(c->mlp);
   for (auto e : c->ev0) hipEventDestroy(e);
   for (auto e : c->ev1) hipEventDestroy(e);
@@ -4247,6 +4308,7 @@ int ndsp_batch_create(ndsp_ctx* const* ctxs, int32_t nsim, ndsp_batch** out) {
   }
   ndsp_batch* b = new ndsp_batch();
   b->sims.assign(ctxs, ctxs + nsim);
+  for (int s = 0; s < nsim; s++) ctxs[s]->in_batch = true;
   b->prec = ctxs[0]->prec;
   bool ok = hipStreamCreate(&b->st) == hipSuccess;
   ok &= hipHostMalloc((void**)&b->hs, sizeof(DScal) * nsim, hipHostMallocDefault) == hipSuccess;
@@ -4394,7 +4456,7 @@ static int restore_state(ndsp_ctx* c, const SavedState& S) {
   c->have_rates = S.have_rates;
   // the restored order may differ from the last binning's
   c->grid_valid = false;
-  c->graph_valid = false;
+  invalidate_graphs(c);
   return 0;
 }
 
@@ -4423,7 +4485,7 @@ int ndsp_set_tile_shapes(ndsp_ctx* c, int32_t force_wg, int32_t density_wg, int3
   c->tb_nr = density_wg;
   c->p.cell_sub = cells_per_radius;
   c->cap_bits = 0;  // the key space follows the cell size at the next binning
-  c->graph_valid = false;  // recorded steps hold the old launch shapes
+  invalidate_graphs(c);  // recorded steps hold the old launch shapes
   c->reorder_next = true;  // the slot order follows the new cells
   return 0;
 }
@@ -4473,7 +4535,7 @@ int ndsp_autotune(ndsp_ctx* c, int32_t reps, int32_t* choice) {
   c->tb_nr = bn;
   c->p.cell_sub = bc;
   c->cap_bits = 0;  // the key space follows the chosen cell size at the next binning
-  c->graph_valid = false;
+  invalidate_graphs(c);
   c->reorder_next = true;  // the restored slot order predates the chosen cells
   const int re = restore_state(c, S);
   free_state(S);
@@ -4612,7 +4674,7 @@ int ndsp_set_ireal(ndsp_ctx* c, const int32_t* ireal) {
 int ndsp_set_graph(ndsp_ctx* c, int32_t on) {
   if (!c) return set_err(NDSP_ERR_BAD_ARG, "null context");
   c->use_graph = on != 0;
-  c->graph_valid = false;
+  invalidate_graphs(c);
   return 0;
 }
 

@@ -33,10 +33,12 @@ def _run_pair(o, st, prec, nsteps):
 
 @pytest.mark.parametrize("prec", [0, 1])
 def test_mhd_box_graph_replay(prec):
+    """23 steps: the reorders at steps 0, 10 and 20 move the state between the
+    two field sets, so the third replays the graphs recorded after the first"""
     from ndspmhd_amd.setups import periodic_mhd_box
     o, st = periodic_mhd_box(12, seed=3, vel_seed=4, vrms=0.5)
     print("mhd box 12^3 graph replay prec", prec, flush=True)
-    _run_pair(o, st, prec, 12)
+    _run_pair(o, st, prec, 23)
 
 
 def test_dust_box_graph_replay():
