@@ -213,7 +213,10 @@ int ndsp_set_graph(ndsp_ctx *ctx, int32_t on);
  * keeps its particles unsorted): every `every` steps (0 = never; default 10)
  * the evolved state is permuted into the cell order of the last binning, so
  * that per-particle reads and writes of the pair kernels stay coalesced.
- * Results differ from another interval only in summation order. */
+ * Results differ from another interval only in summation order.  The first
+ * reorder of a context outside a batch allocates a second set of the
+ * persistent fields (62 doubles per particle) that the state then moves
+ * between; when that allocation fails the state is copied back instead. */
 int ndsp_set_reorder(ndsp_ctx *ctx, int32_t every);
 
 /* ---- batches of independent simulations (a parameter sweep; the reference
