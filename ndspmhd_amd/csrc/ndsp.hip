@@ -3600,10 +3600,12 @@ int ndsp_create(const int32_t* iopt, const double* dopt, int64_t npart, int32_t 
   if (ok) ok &= hipMemset(a.nrflag, 0, sizeof(int) * n) == hipSuccess;
   ok &= hipMalloc(&a.P4, sizeof(double4) * n) == hipSuccess;
   ok &= hipMalloc(&a.dxb, sizeof(double) * n) == hipSuccess;
-  const size_t v4 = sizeof(double4) * n, v1 = sizeof(double) * n;
-  a.dr_stride = c->prec == NDSP_PREC_FP64 ? dr_bytes<double>(c->p.idust != 0) : dr_bytes<float>(c->p.idust != 0);
+  const size_t v4 = sizeof(double4) * n;
+  const bool fp64 = c->prec == NDSP_PREC_FP64;
+  a.dr_stride = fp64 ? dr_bytes<double>(c->p.idust != 0) : dr_bytes<float>(c->p.idust != 0);
   ok &= hipMalloc(&a.DR, (size_t)a.dr_stride * n) == hipSuccess;
-  ok &= hipMalloc(&a.FR, fr_bytes<double>(fr_phys(c->p)) * n) == hipSuccess;
+  const size_t fr_rec = fp64 ? fr_bytes<double>(fr_phys(c->p)) : fr_bytes<float>(fr_phys(c->p));
+  ok &= hipMalloc(&a.FR, fr_rec * n) == hipSuccess;
   if (c->p.fixed_on) ok &= hipMalloc(&a.GF, v4) == hipSuccess;
   ok &= hipMalloc((void**)&a.sc, sizeof(DScal)) == hipSuccess;
   a.rcap = (int)nblk(n, 64);
