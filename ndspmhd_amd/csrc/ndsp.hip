@@ -1230,49 +1230,64 @@ __device__ __forceinline__ void block_partials(double (&v)[NQ], const int (&kind
 __device__ __forceinline__ void b_red_final(const double* __restrict__ red, int rcap, int nblocks,
                                                   RedSpec spec, DScal* sc, const int* skip, VL vl) {
   if (skip && *skip) return;
-  __shared__ double sm[BS];
-  for (int q = 0; q < spec.nq; q++) {
-    const int kind = spec.op[q] & 15, type = spec.op[q] >> 4;
-    const double neutral = kind == RED_SUM ? 0.0 : kind == RED_MAX ? -DBL_MAX : DBL_MAX;
-    // eight independent partial folds per thread keep eight loads in flight
-    // (every fold here is a min, a max or a sum of integers: exact in any order)
-    const double* __restrict__ rq = red + (size_t)q * rcap;
-    double xs[8];
+  // every quantity at once: each thread folds its blocks for all of them
+  // (eight blocks a step, so that up to 8 x nq loads are in flight), then a
+  // butterfly per wave and the waves in order (every fold here is a min, a
+  // max or a sum of integers: exact in any order)
+  __shared__ double sw[BS / 64][RED_MAXQ];
+  const int nq = spec.nq;
+  int kind[RED_MAXQ];
+  double acc[RED_MAXQ];
 #pragma unroll
-    for (int u = 0; u < 8; u++) xs[u] = neutral;
-    int b = threadIdx.x;
-    for (; b + 7 * BS < nblocks; b += 8 * BS) {
+  for (int q = 0; q < RED_MAXQ; q++) {
+    kind[q] = q < nq ? (spec.op[q] & 15) : RED_SUM;
+    acc[q] = kind[q] == RED_SUM ? 0.0 : kind[q] == RED_MAX ? -DBL_MAX : DBL_MAX;
+  }
+  int b = threadIdx.x;
+  for (; b + 7 * BS < nblocks; b += 8 * BS) {
+#pragma unroll
+    for (int q = 0; q < RED_MAXQ; q++) {
+      if (q >= nq) continue;
+      const double* __restrict__ rq = red + (size_t)q * rcap + b;
       double v[8];
 #pragma unroll
-      for (int u = 0; u < 8; u++) v[u] = rq[b + u * BS];
+      for (int u = 0; u < 8; u++) v[u] = rq[u * BS];
 #pragma unroll
-      for (int u = 0; u < 8; u++) xs[u] = red_combine(kind, xs[u], v[u]);
+      for (int u = 0; u < 8; u++) acc[q] = red_combine(kind[q], acc[q], v[u]);
     }
-    for (; b < nblocks; b += BS) xs[0] = red_combine(kind, xs[0], rq[b]);
-    double x = xs[0];
+  }
+  for (; b < nblocks; b += BS) {
 #pragma unroll
-    for (int u = 1; u < 8; u++) x = red_combine(kind, x, xs[u]);
-    sm[threadIdx.x] = x;
-    __syncthreads();
-    for (int s = BS / 2; s > 0; s >>= 1) {
-      if ((int)threadIdx.x < s) sm[threadIdx.x] = red_combine(kind, sm[threadIdx.x], sm[threadIdx.x + s]);
-      __syncthreads();
+    for (int q = 0; q < RED_MAXQ; q++)
+      if (q < nq) acc[q] = red_combine(kind[q], acc[q], red[(size_t)q * rcap + b]);
+  }
+#pragma unroll
+  for (int q = 0; q < RED_MAXQ; q++) {
+    double x = acc[q];
+    for (int o = 32; o > 0; o >>= 1) x = red_combine(kind[q], x, __shfl_xor(x, o, 64));
+    acc[q] = x;
+  }
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int q = 0; q < RED_MAXQ; q++) sw[threadIdx.x >> 6][q] = acc[q];
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < nq) {
+    const int q = threadIdx.x;
+    const int kq = spec.op[q] & 15, type = spec.op[q] >> 4;
+    double r = sw[0][q];
+    for (int w = 1; w < BS / 64; w++) r = red_combine(kq, r, sw[w][q]);
+    char* base = (char*)sc + spec.off[q];
+    if (type == RED_F64) {
+      double* d = (double*)base;
+      *d = red_combine(kq, *d, r);
+    } else if (type == RED_I32) {
+      int* d = (int*)base;
+      *d = (int)red_combine(kq, (double)*d, r);
+    } else {
+      unsigned long long* d = (unsigned long long*)base;
+      *d = kq == RED_SUM ? *d + (unsigned long long)r : (unsigned long long)red_combine(kq, (double)*d, r);
     }
-    if (threadIdx.x == 0) {
-      char* base = (char*)sc + spec.off[q];
-      const double r = sm[0];
-      if (type == RED_F64) {
-        double* d = (double*)base;
-        *d = red_combine(kind, *d, r);
-      } else if (type == RED_I32) {
-        int* d = (int*)base;
-        *d = (int)red_combine(kind, (double)*d, r);
-      } else {
-        unsigned long long* d = (unsigned long long*)base;
-        *d = kind == RED_SUM ? *d + (unsigned long long)r : (unsigned long long)red_combine(kind, (double)*d, r);
-      }
-    }
-    __syncthreads();
   }
 }
 __global__ __launch_bounds__(BS) void k_red_final(const double* __restrict__ red, int rcap, int nblocks,
