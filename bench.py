@@ -317,8 +317,8 @@ def main():
     if args.shapes != "auto":
         SHAPES = tuple(int(v) for v in args.shapes.split(","))
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; multi-GPU sharding is "
-              "a design only. Run with --gpus 1.", file=sys.stderr)
+        print("bench.py: the library runs on one GPU only; sharding over several "
+              "GPUs is a design, not implemented. Run with --gpus 1.", file=sys.stderr)
         return 2
 
     log("[bench] mixed precision, %d^3, warmup %d, steps %d" % (args.nside, args.warmup, args.steps))
